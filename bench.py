@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""Throughput of the LJ cluster-nucleation step on one MI355X.
+
+Default workload (ensemble512): 512 independent Nose-Hoover trajectories,
+N log-uniform in 500..32768 (3.95 M atoms), rho* U(0.01,0.05), T* U(0.6,0.8),
+cutoff 2.5 sigma, list radius 2.8 sigma, dt 0.005 tau, Stillinger analysis
+every 100 steps with first-passage time to a 50-atom cluster.  A "step" is
+one integrator step of every trajectory (rebuilds, forces, thermostat and the
+scheduled cluster analyses included); data are synthetic (seeded vapour).
+
+Reported:
+  value            mixed-precision batched path, atom-steps/s
+  fp64_batched     the same run in FP64
+  fp64_plain       the plain FP64 path: one trajectory at a time (sampled)
+  speedup          value / fp64_plain
+  accuracy         RMS relative force error of the mixed path vs FP64 on the
+                   same state; energy-drift ratio mixed/FP64
+  roofline         dominant kernel, HIP events on the engine stream
+  cpu_baseline     the reference Fortran (flang -O3) on the host cores
+
+Usage: python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fortran-cluster-md_amd"))
+
+import fcmd  # noqa: E402
+from fcmd import workloads as W  # noqa: E402
+
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_system(params, states, precision):
+    s = fcmd.System(params, precision)
+    W.setup(s, params, states)
+    return s
+
+
+def hamiltonian(s, params):
+    """Per-trajectory conserved quantity ekin + epot + psi^2 Q/2 + 3NkT s."""
+    out = []
+    for t, p in enumerate(params):
+        sc = s.scalars(t)
+        nc = p.natom if not p.nh_mask else sum(p.num[a] for a in range(p.nsorts) if (p.nh_mask >> a) & 1)
+        out.append(sc.ekin + sc.epot + sc.psi ** 2 * p.Q / 2 + 3 * nc * fcmd.CIKB * p.att * sc.s)
+    return np.array(out)
+
+
+def force_error(params, states, ntraj=4):
+    """RMS relative force error of the mixed path against FP64 on identical
+    states (first ntraj trajectories)."""
+    num = den = 0.0
+    for t in range(min(ntraj, len(params))):
+        a = {}
+        for prec in (fcmd.FP64, fcmd.MIXED):
+            with fcmd.System([params[t]], prec) as s:
+                W.setup(s, [params[t]], [states[t]])
+                a[prec] = s.download()
+        for k in ("ax", "ay", "az"):
+            num += ((a[fcmd.MIXED][k] - a[fcmd.FP64][k]) ** 2).sum()
+            den += (a[fcmd.FP64][k] ** 2).sum()
+    return float(np.sqrt(num / den))
+
+
+# --------------------------------------------------------------------------
+# CPU baseline: the reference's own plugin set (nose build: int-verlet-nh-mav,
+# neighbor-pbc-mav-silent, for-a-lj-n-pbc-m, stoddard-pbc-n-m) compiled from
+# /root/reference by oracle/Makefile, driven by oracle/ref_harness.f.
+def cpu_baseline(params, states, cores, budget_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    builds = sorted(int(d[3:]) for d in os.listdir(os.path.join(po.REF_DIR, "bin"))
+                    if d.startswith("mxa") and os.path.exists(po.harness_path("nose", int(d[3:]))))
+    if not builds:
+        return None
+    order = np.argsort([p.natom for p in params])
+    pick = [int(order[int(round(q))]) for q in np.linspace(0, len(order) - 1, cores)]
+    procs = []
+    tmp = tempfile.mkdtemp()
+    for k, t in enumerate(pick):
+        p, st = params[t], states[t]
+        n = p.natom
+        mxa = next((m for m in builds if m >= n), None)
+        if mxa is None:
+            continue
+        # cost per step: O(N) update + O(N^2) list rebuild every ~12 steps
+        cost = 2e-7 * n + 7.5e-9 * n * n / 2 / 12
+        steps = int(np.clip(budget_s / cost, 10, 4000))
+        case = dict(natom=n, nsorts=p.nsorts, api=st["api"], box=np.array(p.box[:]),
+                    mas=np.array(p.mas[:p.nsorts]), sig=np.array(p.sig[:p.nsorts]),
+                    eps=np.array(p.eps[:p.nsorts]), num=np.array(p.num[:p.nsorts]),
+                    atrcut=p.atrcut, atrskin=p.atrskin, atskin=p.atskin, dt=p.dt, Q=p.Q,
+                    att=p.att, nbtim=p.nbtim, btim=0, atnenum=p.atnenum, ethst_nh=1,
+                    psi=W.nh_initial_psi(p, st), s=0.0, dmms=0.0, dmmax=0.0, dmmav=0.0,
+                    dmmst=0, dmmi=0, **{q: st[q] for q in ("x", "y", "z", "vx", "vy", "vz",
+                                                            "ax", "ay", "az")})
+        w = float(np.float32(p.nbtim))
+        case["dmmh"] = np.full(256, (p.atrskin - p.atrcut) / w)
+        case["dmmsu"] = 256.0 * (p.atrskin - p.atrcut) / w
+        fin = os.path.join(tmp, f"in{k}.bin")
+        po._write_input(fin, case, [(1, 0, 0), (2, 0, 0), (3, 1, steps)])
+        t0 = time.perf_counter()
+        pr = subprocess.Popen([po.harness_path("nose", mxa), fin, os.path.join(tmp, f"out{k}.bin")],
+                              stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                              preexec_fn=po._unlimited_stack)
+        procs.append((pr, t0, n, steps))
+    rate = 0.0
+    sizes = []
+    for pr, t0, n, steps in procs:
+        pr.wait()
+        dt = time.perf_counter() - t0
+        if pr.returncode == 0:
+            rate += n * steps / dt
+            sizes.append(n)
+    return dict(value=rate, unit="atom-steps/s", cores=len(sizes), kind="reference",
+                sample=f"{len(sizes)} of the trajectories (N={sorted(sizes)}), one serial instance "
+                       f"per core, ~{budget_s:.0f} s each, reference nose plugin set, flang -O3; "
+                       "includes the initial O(N^2) list build")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="ensemble512", choices=sorted(W.WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-cores", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "trajectories shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    if fcmd.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device")
+
+    K, Wm = args.steps, args.warmup
+    log(f"[bench] building workload {args.workload}")
+    params, states, meta = W.WORKLOADS[args.workload]()
+    natoms = int(sum(p.natom for p in params))
+    every = meta.get("cluster_every", 0)
+    fpt_size = meta.get("fpt_size", 50)
+
+    res = {}
+    hdrift = {}
+    for name, prec in (("mixed", fcmd.MIXED), ("fp64", fcmd.FP64)):
+        log(f"[bench] {name}: setup {len(params)} trajectories, {natoms} atoms")
+        s = build_system(params, states, prec)
+        s.run(Wm, cluster_every=0, use_graph=True)
+        h0 = hamiltonian(s, params)
+        t0 = time.perf_counter()
+        series, fpt = s.run(K, cluster_every=every, fpt_size=fpt_size, use_graph=True)
+        wall = time.perf_counter() - t0
+        h1 = hamiltonian(s, params)
+        ev_ms = s.last_run_ms()
+        tau = K * params[0].dt / W.TAU
+        hdrift[name] = np.abs(h1 - h0) / np.abs(h0) / tau
+        res[name] = dict(wall=wall, ev_ms=ev_ms, rate=natoms * K / wall,
+                         nlist=s.list_entries(), largest=int(series.max()) if series.size else 0,
+                         nfpt=int((fpt >= 0).sum()))
+        log(f"[bench] {name}: {K} steps in {wall:.3f} s (events {ev_ms / 1e3:.3f} s) -> "
+            f"{res[name]['rate']:.4e} atom-steps/s")
+        if name == "mixed":
+            log("[bench] mixed: kernel timing (HIP events)")
+            kt = s.time_kernels(20)
+            res[name]["kernels"] = kt
+            res[name]["nlist_k"] = s.list_entries()
+        s.close()
+
+    plain = None
+    if not args.no_plain and len(params) > 1:
+        # one trajectory at a time, FP64, sampled every 8th trajectory
+        log("[bench] plain FP64 path, one trajectory at a time")
+        kp = max(20, min(K, 100))
+        tot_t = 0.0
+        tot_as = 0.0
+        for t in range(0, len(params), 8):
+            with fcmd.System([params[t]], fcmd.FP64) as s:
+                W.setup(s, [params[t]], [states[t]])
+                s.run(5, use_graph=False)
+                t0 = time.perf_counter()
+                s.run(kp, cluster_every=every if every <= kp else 0, use_graph=False)
+                tot_t += time.perf_counter() - t0
+                tot_as += params[t].natom * kp
+        plain = tot_as / tot_t
+        log(f"[bench] plain FP64: {plain:.4e} atom-steps/s")
+
+    ferr = force_error(params, states)
+    drift_ratio = float(np.median(hdrift["mixed"]) / max(np.median(hdrift["fp64"]), 1e-300))
+
+    # roofline: the dominant kernel of the mixed run, algorithmic bytes per
+    # atom per launch (x_i, v_i, a_i write, list entries; neighbour gathers
+    # are cache traffic and not counted)
+    kt = res["mixed"]["kernels"]
+    nl = res["mixed"]["nlist_k"] / natoms
+    per_atom = {"drift": 24 + 24 + 12 + 24 + 24, "force": 24 + 24 + 12 + 4 + 4 * nl,
+                "kick": 24 + 12 + 24, "rebuild": 0.0}
+    dom = max(("drift", "force", "kick"), key=lambda k: kt[k])
+    bytes_launch = per_atom[dom] * natoms
+    achieved = bytes_launch / (kt[dom] * 1e-3)
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
+    if os.path.exists(tp):
+        traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
+    step_bytes = (120 + 4 * nl) * natoms  # whole-step algorithmic bytes (mixed)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (reference Fortran on host cores)")
+        try:
+            cpu = cpu_baseline(params, states, args.cpu_cores)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] CPU baseline unavailable: {e}")
+        if cpu:
+            log(f"[bench] CPU baseline: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} cores")
+
+    value = res["mixed"]["rate"]
+    out = {
+        "metric": "atom-steps/s: mixed-prec batched vs plain FP64 on 1 GPU, <=1e-5 force err",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": res["mixed"]["wall"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair math + fp64 state/reductions (mixed); fp64 for fp64_* fields",
+        "data": "synthetic: seeded LJ vapour (uniform, 0.9 sigma minimum, Maxwell-Boltzmann)",
+        "config": {"workload": args.workload, "trajectories": len(params), "atoms": natoms,
+                   "cluster_every": every, "fpt_size": fpt_size, "parallelism": "1 GPU"},
+        "fp64_batched": res["fp64"]["rate"],
+        "fp64_plain": plain,
+        "speedup_vs_fp64_plain": value / plain if plain else None,
+        "speedup_vs_fp64_batched": value / res["fp64"]["rate"],
+        "rms_rel_force_err": ferr,
+        "energy_drift_ratio_mixed_over_fp64": drift_ratio,
+        "largest_cluster": res["mixed"]["largest"],
+        "trajectories_reaching_fpt": res["mixed"]["nfpt"],
+        "kernel_ms_per_step": kt,
+        "step_roofline_frac": step_bytes * K / res["mixed"]["wall"] / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "bytes_per_atom": per_atom[dom], "list_entries_per_atom": nl},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

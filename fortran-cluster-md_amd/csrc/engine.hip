@@ -1,0 +1,803 @@
+// engine.hip -- host side of the engine and the C ABI (include/fcmd.h).
+//
+// Owns the device arena (fcmd_dev.hpp), turns the reference's parameters into
+// device constants with the reference's operand order, maps the drop-in entry
+// points onto kernel sequences, and replays a captured step graph for the
+// ensemble run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/fcmd.h"
+#include "fcmd_kernels.hpp"
+
+using namespace fcmd;
+
+namespace {
+
+constexpr double CIKB = 1.38066e-6;  // const.inc:17
+
+long long roundup(long long v, long long m) { return (v + m - 1) / m * m; }
+
+#define HIPCHK(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            std::fprintf(stderr, "fcmd: %s failed: %s\n", #call, hipGetErrorString(e_)); \
+            return FCMD_ERR_HIP;                                                     \
+        }                                                                            \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// small kernels of the host interface
+__global__ void k_scatter_d(double *dst, const double *src, const int *inv, int abase, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[inv[abase + k]] = src[k];
+}
+__global__ void k_gather_d(double *dst, const double *src, const int *inv, int abase, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[inv[abase + k]];
+}
+__global__ void k_scatter_f(float *dst, const double *src, const int *inv, int abase, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[inv[abase + k]] = (float)src[k];
+}
+__global__ void k_gather_f(double *dst, const float *src, const int *inv, int abase, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = (double)src[inv[abase + k]];
+}
+__global__ void k_scatter_typ(int *typ, const int *api, const int *inv, int abase, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) typ[inv[abase + k]] = api[k] - 1;
+}
+__global__ void k_init_slots(int *oid, int *inv, int *typ, int n_total, const int *blk_traj,
+                             const TrajParams *tp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_total) return;
+    const TrajParams &p = tp[blk_traj[i / BLK]];
+    const int rel = i - p.abase;
+    oid[i] = rel < p.natom ? rel + 1 : 0;
+    inv[i] = i;
+    if (rel >= p.natom) typ[i] = -1;
+}
+// per-trajectory: record the largest cluster of this analysis and the first
+// passage to fpt_size
+__global__ void k_record(const ClusterScalars *cls, const TrajScalars *ts, int ntraj, int *series,
+                         int *fpt, int fpt_size, int step)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntraj) return;
+    const int big = cls[t].cbig;
+    if (series) series[t] = big;
+    if (fpt && fpt[t] < 0 && big >= fpt_size) fpt[t] = step;
+}
+
+}  // namespace
+
+namespace fcmd {
+void launch_neighbor_entry(const DevPtrs &d, int traj, int ntraj, int step, int *step_out,
+                           hipStream_t s);
+}
+
+// ---------------------------------------------------------------------------
+struct fcmd_system {
+    int ntraj = 0;
+    bool mixed = false;
+    int list_width = 128;
+    std::vector<fcmd_params> hp;
+    std::vector<TrajParams> tp;
+    LaunchDims g{};
+    long long natoms_total = 0, ncells_total = 0, ntiles_total = 0, nlist_total = 0, nhist_total = 0;
+    bool single_species = true;
+    bool any_fre = false;
+    DevPtrs d{};
+    std::vector<void *> allocs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *stage = nullptr;    // natom-sized staging (device)
+    int *stage_i = nullptr;
+    int *dstep = nullptr;       // device int for step round trips
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    double last_ms = 0.0;
+
+    template <typename T>
+    int alloc(T **p, long long n)
+    {
+        void *q = nullptr;
+        const size_t bytes = (size_t)std::max<long long>(n, 1) * sizeof(T);
+        if (hipMalloc(&q, bytes) != hipSuccess) return FCMD_ERR_HIP;
+        if (hipMemset(q, 0, bytes) != hipSuccess) return FCMD_ERR_HIP;
+        allocs.push_back(q);
+        *p = (T *)q;
+        return FCMD_OK;
+    }
+    ~fcmd_system()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void *p : allocs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// pair constants with the reference's operand order
+void pair_consts(const fcmd_params &p, TrajParams &t)
+{
+    auto six = [](double s) { return s * s * s * s * s * s; };
+    const int ns = std::max(1, p.nsorts);
+    for (int a = 0; a < MAXS; ++a)
+        for (int b = 0; b < MAXS; ++b) {
+            const int k = a * MAXS + b;
+            double c1 = 0, eps = 0;
+            if (a < ns && b < ns) {
+                if (p.accatom == FCMD_ACC_TABLE) {
+                    c1 = six(p.sig_ab[k]);
+                    eps = p.eps_ab[k];
+                } else if (a == b) {
+                    c1 = six(p.sig[a]);  // for-a-lj-n-pbc-m.f:324, :329
+                    eps = p.eps[a];
+                } else {
+                    c1 = six(0.5 * (p.sig[a] + p.sig[b]));  // :334-335 Lorentz-Berthelot
+                    eps = std::sqrt(p.eps[a] * p.eps[b]);   // :338
+                }
+            }
+            t.c1[k] = c1;
+            t.c2[k] = c1 * c1;
+            t.c3[k] = 0.5 * c1;
+            t.e48[k] = eps * 48.0;
+            t.e4[k] = eps * 4.0;
+            t.c1f[k] = (float)t.c1[k];
+            t.c2f[k] = (float)t.c2[k];
+            t.c3f[k] = (float)t.c3[k];
+        }
+}
+
+int fill_params(fcmd_system *s, int k, const fcmd_params &p, TrajParams &t)
+{
+    if (p.natom <= 0 || p.nsorts < 1 || p.nsorts > MAXS) return FCMD_ERR_ARG;
+    for (int d = 0; d < 3; ++d) {
+        if (!(p.box[d] > 0)) return FCMD_ERR_ARG;
+        t.box[d] = p.box[d];
+        t.ibox[d] = 1.0 / p.box[d];
+    }
+    t.atrcut = p.atrcut;
+    t.atskin = p.atskin;
+    t.atr = p.sig[0];
+    t.eps1 = p.eps[0];
+    t.dt = p.dt;
+    t.Q = p.Q;
+    t.att = p.att;
+    for (int a = 0; a < MAXS; ++a) t.mas[a] = a < p.nsorts ? p.mas[a] : 1.0;
+    pair_consts(p, t);
+    t.natom = p.natom;
+    t.nsorts = p.nsorts;
+    t.nbtim = p.nbtim;
+    t.btim = p.btim;
+    t.atnenum = p.atnenum;
+    t.ethst_nh = p.ethst_nh;
+    t.integrator = p.integrator;
+    t.neighbor = p.neighbor;
+    t.accatom = p.accatom;
+    t.stoddard = p.stoddard;
+    if (p.nh_mask == 0) {
+        t.nh_mask = (1 << MAXS) - 1;
+        t.nh_refmass = 1;
+        t.nkt = 3.0 * p.natom * CIKB * p.att;  // int-verlet-nh-mav.f:65
+    } else {
+        t.nh_mask = p.nh_mask;
+        t.nh_refmass = 0;
+        int nc = 0;
+        for (int a = 0; a < p.nsorts; ++a)
+            if ((p.nh_mask >> a) & 1) nc += p.num[a];
+        t.nkt = 3.0 * nc * CIKB * p.att;
+    }
+    (void)s;
+    (void)k;
+    return FCMD_OK;
+}
+
+int upload_params(fcmd_system *s)
+{
+    HIPCHK(hipMemcpy(s->d.tp, s->tp.data(), sizeof(TrajParams) * s->ntraj, hipMemcpyHostToDevice));
+    return FCMD_OK;
+}
+
+int check_traj(fcmd_system *s, int traj) { return (s && traj >= 0 && traj < s->ntraj) ? FCMD_OK : FCMD_ERR_ARG; }
+
+int sync_err(fcmd_system *s)
+{
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::vector<TrajScalars> ts(s->ntraj);
+    HIPCHK(hipMemcpy(ts.data(), s->d.ts, sizeof(TrajScalars) * s->ntraj, hipMemcpyDeviceToHost));
+    for (auto &t : ts)
+        if (t.err) return t.err;
+    return FCMD_OK;
+}
+
+// one integrator step of every trajectory
+void enqueue_step(fcmd_system *s)
+{
+    launch_drift(s->d, s->g.nblk, s->mixed, s->stream);
+    launch_rebuild(s->d, s->g, s->mixed, s->stream);
+    launch_force(s->d, s->g.nblk, s->mixed, s->single_species, FORCE_STEP, s->stream);
+    launch_kick_nh(s->d, s->g.nblk, s->mixed, s->stream);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fcmd_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *fcmd_error_text(int code)
+{
+    switch (code) {
+    case FCMD_OK: return "ok";
+    case FCMD_ERR_SKIN: return "*** FATAL ERROR! ***  Skin radius exceeded!!";
+    case FCMD_ERR_LISTFULL: return "*** FATAL ERROR *** Neighborlist is nearly full!";
+    case FCMD_ERR_NH: return "Numerical integration did not converge!";
+    case FCMD_ERR_EDGES: return "*** FATAL ERROR *** too many cluster neighbours per atom";
+    case FCMD_ERR_CELLS: return "*** FATAL ERROR *** cell grid exceeds its allocation";
+    case FCMD_ERR_ARG: return "invalid argument";
+    case FCMD_ERR_HIP: return "HIP runtime error";
+    case FCMD_ERR_NODEVICE: return "no HIP device";
+    default: return "unknown error";
+    }
+}
+
+int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int precision, int list_width)
+{
+    if (!out || ntraj <= 0 || !params) return FCMD_ERR_ARG;
+    *out = nullptr;
+    if (fcmd_device_count() <= 0) return FCMD_ERR_NODEVICE;
+    fcmd_system *s = new (std::nothrow) fcmd_system();
+    if (!s) return FCMD_ERR_ARG;
+    s->ntraj = ntraj;
+    s->mixed = precision == FCMD_MIXED;
+    s->list_width = list_width > 0 ? list_width : 128;
+    s->hp.assign(params, params + ntraj);
+    s->tp.resize(ntraj);
+    int rc;
+    long long abase = 0, cbase = 0, tbase = 0, lbase = 0, hbase = 0;
+    for (int k = 0; k < ntraj; ++k) {
+        const fcmd_params &p = params[k];
+        TrajParams &t = s->tp[k];
+        std::memset(&t, 0, sizeof(t));
+        if ((rc = fill_params(s, k, p, t)) != FCMD_OK) {
+            delete s;
+            return rc;
+        }
+        if (p.nsorts > 1 || p.accatom != FCMD_ACC_1) s->single_species = false;
+        if (p.stoddard == FCMD_ST_FRE) s->any_fre = true;
+        const long long npad = roundup(p.natom, BLK);
+        // smallest cell side the list radius can reach: min(atrskin, atrcut)
+        const double smin = std::min(p.atrskin, p.atrcut) * (1.0 - 1e-6);
+        long long nc = 1;
+        for (int d = 0; d < 3; ++d) nc *= std::max<long long>(1, (long long)std::floor(p.box[d] / smin));
+        const long long ncpad = roundup(nc, SCAN_CHUNK);
+        const long long ntile = npad / TILE;
+        const long long ntpad = roundup(ntile, SCAN_CHUNK);
+        t.abase = (int)abase;
+        t.npad = (int)npad;
+        t.bfirst = (int)(abase / BLK);
+        t.nblk = (int)(npad / BLK);
+        t.cbase = (int)cbase;
+        t.ncpad = (int)ncpad;
+        t.cfirst = (int)(cbase / SCAN_CHUNK);
+        t.ncblk = (int)(ncpad / SCAN_CHUNK);
+        t.tbase = (int)tbase;
+        t.ntile = (int)ntile;
+        t.tfirst = (int)(tbase / SCAN_CHUNK);
+        t.ntblk = (int)(ntpad / SCAN_CHUNK);
+        t.lbase = lbase;
+        t.lcap = npad * s->list_width;
+        t.hbase = (int)hbase;
+        abase += npad;
+        cbase += ncpad;
+        tbase += ntpad;
+        lbase += t.lcap;
+        hbase += p.natom + 1;
+    }
+    if (abase >= (1LL << 31) || cbase >= (1LL << 31) || lbase >= (1LL << 40)) {
+        delete s;
+        return FCMD_ERR_ARG;
+    }
+    s->natoms_total = abase;
+    s->ncells_total = cbase;
+    s->ntiles_total = tbase;
+    s->nlist_total = lbase;
+    s->nhist_total = hbase;
+    s->g.nblk = (int)(abase / BLK);
+    s->g.ncblk = (int)(cbase / SCAN_CHUNK);
+    s->g.ntblk = (int)(tbase / SCAN_CHUNK);
+    s->g.ntraj = ntraj;
+
+    DevPtrs &d = s->d;
+    const long long N = abase;
+#define A(ptr, n)                                  \
+    if ((rc = s->alloc(&(ptr), (n))) != FCMD_OK) { \
+        delete s;                                  \
+        return rc;                                 \
+    }
+    A(d.x, N) A(d.y, N) A(d.z, N) A(d.vx, N) A(d.vy, N) A(d.vz, N);
+    A(d.x2, N) A(d.y2, N) A(d.z2, N) A(d.vx2, N) A(d.vy2, N) A(d.vz2, N);
+    if (s->mixed) {
+        A(d.axf, N) A(d.ayf, N) A(d.azf, N) A(d.axf2, N) A(d.ayf2, N) A(d.azf2, N);
+    } else {
+        A(d.ax, N) A(d.ay, N) A(d.az, N) A(d.ax2, N) A(d.ay2, N) A(d.az2, N);
+    }
+    A(d.typ, N) A(d.oid, N) A(d.typ2, N) A(d.oid2, N) A(d.inv, N);
+    A(d.blk_traj, s->g.nblk) A(d.cblk_traj, s->g.ncblk) A(d.tblk_traj, s->g.ntblk);
+    A(d.tp, ntraj) A(d.ts, ntraj) A(d.cls, ntraj);
+    A(d.part, (long long)s->g.nblk * NPART);
+    A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
+    A(d.scan_tmp_ll, std::max(s->g.ncblk, s->g.ntblk));
+    A(d.nn, N) A(d.tile_w, tbase) A(d.tile_off, tbase) A(d.list, lbase);
+    A(d.edeg, N) A(d.edeg1, N) A(d.edges, N * EMAX) A(d.parent, N) A(d.parentf, N) A(d.liq, N);
+    A(d.cs_tmp, N) A(d.cgs_tmp, N) A(d.csf_tmp, N) A(d.cek_tmp, N) A(d.cekf_tmp, N);
+    A(d.liqo, N) A(d.rank, N) A(d.o_cl, N) A(d.o_cs, N) A(d.o_cgs, N) A(d.o_clf, N) A(d.o_ccf, N);
+    A(d.o_cek, N);
+    A(d.o_csi, hbase) A(d.o_csf, hbase) A(d.o_csif, hbase) A(d.o_cgsf, hbase) A(d.o_cgsfa, hbase);
+    A(d.o_ceks, hbase) A(d.o_cekf, hbase) A(d.o_ceksf, hbase);
+    long long maxn = 1;
+    for (auto &p : s->hp) maxn = std::max<long long>(maxn, p.natom + 1);
+    A(s->stage, maxn) A(s->stage_i, maxn) A(s->dstep, 4);
+#undef A
+
+    std::vector<int> bt(s->g.nblk), ct(s->g.ncblk), tt(s->g.ntblk);
+    for (int k = 0; k < ntraj; ++k) {
+        const TrajParams &t = s->tp[k];
+        for (int b = 0; b < t.nblk; ++b) bt[t.bfirst + b] = k;
+        for (int b = 0; b < t.ncblk; ++b) ct[t.cfirst + b] = k;
+        for (int b = 0; b < t.ntblk; ++b) tt[t.tfirst + b] = k;
+    }
+    if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
+        hipEventCreate(&s->ev1) != hipSuccess) {
+        delete s;
+        return FCMD_ERR_HIP;
+    }
+    if (hipMemcpy(d.blk_traj, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.cblk_traj, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.tblk_traj, tt.data(), sizeof(int) * tt.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        upload_params(s) != FCMD_OK) {
+        delete s;
+        return FCMD_ERR_HIP;
+    }
+    // scalars: reference start values (read-3d.f:263-268), psi = s = 0
+    std::vector<TrajScalars> ts(ntraj);
+    for (int k = 0; k < ntraj; ++k) {
+        const fcmd_params &p = params[k];
+        TrajScalars &t = ts[k];
+        std::memset(&t, 0, sizeof(t));
+        t.atrskin = p.atrskin;
+        const double w = (double)(float)p.nbtim;
+        for (int i = 0; i < 256; ++i) t.dmmh[i] = (p.atrskin - p.atrcut) / w;
+        t.dmmsu = 256.0 * (p.atrskin - p.atrcut) / w;
+        t.step = p.btim;
+    }
+    if (hipMemcpy(d.ts, ts.data(), sizeof(TrajScalars) * ntraj, hipMemcpyHostToDevice) != hipSuccess) {
+        delete s;
+        return FCMD_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_init_slots, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s->stream, d.oid,
+                       d.inv, d.typ, (int)N, d.blk_traj, d.tp);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) {
+        delete s;
+        return FCMD_ERR_HIP;
+    }
+    *out = s;
+    return FCMD_OK;
+}
+
+void fcmd_destroy(fcmd_system *s) { delete s; }
+
+int fcmd_set_params(fcmd_system *s, int traj, const fcmd_params *p)
+{
+    int rc;
+    if ((rc = check_traj(s, traj)) || !p) return FCMD_ERR_ARG;
+    if (p->natom != s->hp[traj].natom) return FCMD_ERR_ARG;
+    TrajParams t = s->tp[traj];
+    if ((rc = fill_params(s, traj, *p, t)) != FCMD_OK) return rc;
+    s->hp[traj] = *p;
+    s->tp[traj] = t;
+    if (p->nsorts > 1 || p->accatom != FCMD_ACC_1) s->single_species = false;
+    if (p->stoddard == FCMD_ST_FRE) s->any_fre = true;
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    return upload_params(s);
+}
+
+int fcmd_upload(fcmd_system *s, int traj, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *ax,
+                const double *ay, const double *az, const int *api)
+{
+    if (check_traj(s, traj)) return FCMD_ERR_ARG;
+    const TrajParams &t = s->tp[traj];
+    const int n = t.natom;
+    const dim3 grid((n + 255) / 256), blk(256);
+    const double *src[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+    double *dd[9] = {s->d.x, s->d.y, s->d.z, s->d.vx, s->d.vy, s->d.vz, s->d.ax, s->d.ay, s->d.az};
+    float *df[3] = {s->d.axf, s->d.ayf, s->d.azf};
+    for (int k = 0; k < 9; ++k) {
+        if (!src[k]) continue;
+        HIPCHK(hipMemcpyAsync(s->stage, src[k], sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+        if (k >= 6 && s->mixed)
+            hipLaunchKernelGGL(k_scatter_f, grid, blk, 0, s->stream, df[k - 6], s->stage, s->d.inv, t.abase, n);
+        else
+            hipLaunchKernelGGL(k_scatter_d, grid, blk, 0, s->stream, dd[k], s->stage, s->d.inv, t.abase, n);
+        HIPCHK(hipGetLastError());
+    }
+    if (api) {
+        HIPCHK(hipMemcpyAsync(s->stage_i, api, sizeof(int) * n, hipMemcpyHostToDevice, s->stream));
+        hipLaunchKernelGGL(k_scatter_typ, grid, blk, 0, s->stream, s->d.typ, s->stage_i, s->d.inv, t.abase, n);
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return FCMD_OK;
+}
+
+int fcmd_download(fcmd_system *s, int traj, double *x, double *y, double *z, double *vx, double *vy,
+                  double *vz, double *ax, double *ay, double *az)
+{
+    if (check_traj(s, traj)) return FCMD_ERR_ARG;
+    const TrajParams &t = s->tp[traj];
+    const int n = t.natom;
+    const dim3 grid((n + 255) / 256), blk(256);
+    double *dst[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+    double *dd[9] = {s->d.x, s->d.y, s->d.z, s->d.vx, s->d.vy, s->d.vz, s->d.ax, s->d.ay, s->d.az};
+    float *df[3] = {s->d.axf, s->d.ayf, s->d.azf};
+    for (int k = 0; k < 9; ++k) {
+        if (!dst[k]) continue;
+        if (k >= 6 && s->mixed)
+            hipLaunchKernelGGL(k_gather_f, grid, blk, 0, s->stream, s->stage, df[k - 6], s->d.inv, t.abase, n);
+        else
+            hipLaunchKernelGGL(k_gather_d, grid, blk, 0, s->stream, s->stage, dd[k], s->d.inv, t.abase, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dst[k], s->stage, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return FCMD_OK;
+}
+
+int fcmd_set_scalars(fcmd_system *s, int traj, const fcmd_scalars *sc)
+{
+    if (check_traj(s, traj) || !sc) return FCMD_ERR_ARG;
+    TrajScalars t;
+    HIPCHK(hipMemcpy(&t, s->d.ts + traj, sizeof(t), hipMemcpyDeviceToHost));
+    t.atrskin = sc->atrskin;
+    t.psi = sc->psi;
+    t.s = sc->s;
+    t.ekin = sc->ekin;
+    t.epot = sc->epot;
+    t.sumv2 = sc->sumv2;
+    std::memcpy(t.dmmh, sc->dmmh, sizeof(t.dmmh));
+    t.dmms = sc->dmms;
+    t.dmmsu = sc->dmmsu;
+    t.dmmax = sc->dmmax;
+    t.dmmav = sc->dmmav;
+    t.dmmst = sc->dmmst;
+    t.dmmi = sc->dmmi;
+    t.step = sc->step;
+    t.err = sc->err;
+    HIPCHK(hipMemcpy(s->d.ts + traj, &t, sizeof(t), hipMemcpyHostToDevice));
+    return FCMD_OK;
+}
+
+int fcmd_get_scalars(fcmd_system *s, int traj, fcmd_scalars *sc)
+{
+    if (check_traj(s, traj) || !sc) return FCMD_ERR_ARG;
+    TrajScalars t;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(&t, s->d.ts + traj, sizeof(t), hipMemcpyDeviceToHost));
+    sc->atrskin = t.atrskin;
+    sc->psi = t.psi;
+    sc->s = t.s;
+    sc->ekin = t.ekin;
+    sc->epot = t.epot;
+    sc->sumv2 = t.sumv2;
+    std::memcpy(sc->dmmh, t.dmmh, sizeof(t.dmmh));
+    sc->dmms = t.dmms;
+    sc->dmmsu = t.dmmsu;
+    sc->dmmax = t.dmmax;
+    sc->dmmav = t.dmmav;
+    sc->dmmsr = t.dmmsr;
+    sc->dmmst = t.dmmst;
+    sc->dmmi = t.dmmi;
+    sc->step = t.step;
+    sc->npairs = t.npairs;
+    sc->nrebuild = t.nrebuild;
+    sc->nh_iters = t.nh_iters;
+    sc->err = t.err;
+    sc->nlist = t.nlist;
+    return FCMD_OK;
+}
+
+int fcmd_neighbor(fcmd_system *s, int traj, int *step)
+{
+    if (check_traj(s, traj) || !step) return FCMD_ERR_ARG;
+    launch_neighbor_entry(s->d, traj, s->ntraj, *step, s->dstep, s->stream);
+    launch_rebuild(s->d, s->g, s->mixed, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(step, s->dstep, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    return sync_err(s);
+}
+
+int fcmd_accatom(fcmd_system *s)
+{
+    if (!s) return FCMD_ERR_ARG;
+    launch_force(s->d, s->g.nblk, s->mixed, s->single_species, FORCE_ACCATOM, s->stream);
+    HIPCHK(hipGetLastError());
+    return sync_err(s);
+}
+
+int fcmd_integrator(fcmd_system *s, int *step)
+{
+    if (!s || !step) return FCMD_ERR_ARG;
+    // the reference passes integrator(i) the step number i
+    for (int k = 0; k < s->ntraj; ++k)
+        HIPCHK(hipMemcpyAsync((char *)(s->d.ts + k) + offsetof(TrajScalars, step), step, sizeof(int),
+                              hipMemcpyHostToDevice, s->stream));
+    enqueue_step(s);
+    HIPCHK(hipGetLastError());
+    return sync_err(s);
+}
+
+int fcmd_stoddard(fcmd_system *s, int traj, int step, int *cl, int *cs, int *cc, int *csi, double *cek,
+                  double *ceks, int *clf, int *csf, int *ccf, int *csif, double *cekf, double *ceksf,
+                  int *cgs, int *cgsf, int *cgsfa, int *cfpt, int *cfptf, int *counters)
+{
+    if (check_traj(s, traj)) return FCMD_ERR_ARG;
+    const TrajParams &t = s->tp[traj];
+    const bool fre = t.stoddard == ST_FRE;
+    const int n = t.natom;
+    std::vector<ClusterScalars> cls(s->ntraj);
+    HIPCHK(hipMemcpy(cls.data(), s->d.cls, sizeof(ClusterScalars) * s->ntraj, hipMemcpyDeviceToHost));
+    for (int k = 0; k < s->ntraj; ++k) cls[k].want = (k == traj);
+    HIPCHK(hipMemcpy(s->d.cls, cls.data(), sizeof(ClusterScalars) * s->ntraj, hipMemcpyHostToDevice));
+    launch_cluster(s->d, s->g, fre, s->stream);
+    HIPCHK(hipGetLastError());
+    int rc = sync_err(s);
+    if (rc) return rc;
+    ClusterScalars c;
+    HIPCHK(hipMemcpy(&c, s->d.cls + traj, sizeof(c), hipMemcpyDeviceToHost));
+    auto geti = [&](int *dst, const int *src, long long cnt) -> int {
+        if (dst) HIPCHK(hipMemcpy(dst, src, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+        return FCMD_OK;
+    };
+    auto getd = [&](double *dst, const double *src, long long cnt) -> int {
+        if (dst) HIPCHK(hipMemcpy(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+        return FCMD_OK;
+    };
+    const int a = t.abase, h = t.hbase;
+    // csi(0) and ceks(0) are never reset by the reference; keep what the caller had
+    const int csi0 = csi ? csi[0] : 0;
+    const double ceks0 = ceks ? ceks[0] : 0.0;
+    if ((rc = geti(cl, s->d.o_cl + a, n)) || (rc = geti(cs, s->d.o_cs + a, n)) ||
+        (rc = geti(csi, s->d.o_csi + h, n + 1)) || (rc = getd(cek, s->d.o_cek + a, n)) ||
+        (rc = getd(ceks, s->d.o_ceks + h, n + 1)))
+        return rc;
+    if (cc)
+        for (int i = 0; i < n; ++i) cc[i] = 1;
+    if (csi) csi[0] = csi0 + (n - c.npos);  // stoddard-pbc-n.f:145-146 with cs = 0 rows
+    if (ceks) ceks[0] = ceks0;
+    int *cnt = counters;
+    if (cnt) cnt[0] = n;  // cnum: one number per visited atom
+    if (fre) {
+        if ((rc = geti(clf, s->d.o_clf + a, n)) || (rc = geti(ccf, s->d.o_ccf + a, n)) ||
+            (rc = geti(csf, s->d.o_csf + h, n + 1)) || (rc = geti(csif, s->d.o_csif + h, n + 1)) ||
+            (rc = getd(cekf, s->d.o_cekf + h, n + 1)) || (rc = getd(ceksf, s->d.o_ceksf + h, n + 1)) ||
+            (rc = geti(cgs, s->d.o_cgs + a, n)) || (rc = geti(cgsf, s->d.o_cgsf + h, n + 1)) ||
+            (rc = geti(cgsfa, s->d.o_cgsfa + h, n + 1)))
+            return rc;
+        if (csf) csf[0] = 0;
+        if (cekf) cekf[0] = c.cekf0;
+        if (ceksf) ceksf[0] = 0.0;
+        if (cgsf) cgsf[0] = 0;
+        if (cgsfa) cgsfa[0] = 0;
+        if (csif) csif[0] = s->hp[traj].num[0] - c.nliq;  // stoddard-pbc-n-m-fre.f:251
+        if (cnt) {
+            cnt[1] = c.nliq;  // cnumf
+            // first-passage bookkeeping, stoddard-pbc-n-m-fre.f:229-258
+            if (c.cbig > cnt[2]) cnt[2] = c.cbig;
+            if (cnt[2] >= cnt[4]) {
+                if (cfpt)
+                    for (int i = cnt[4]; i <= cnt[2]; ++i) cfpt[i - 1] = step;
+                cnt[4] += 1;
+            }
+            if (c.cbigf > cnt[3]) cnt[3] = c.cbigf;
+            if (cnt[3] >= cnt[5]) {
+                if (cfptf)
+                    for (int i = cnt[5]; i <= cnt[3]; ++i) cfptf[i - 1] = step;
+                cnt[5] += 1;
+            }
+        }
+    }
+    return FCMD_OK;
+}
+
+int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *atnidx, long long *len)
+{
+    if (check_traj(s, traj) || !atnlist || !atnidx || !len) return FCMD_ERR_ARG;
+    const TrajParams &t = s->tp[traj];
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int n = t.natom;
+    std::vector<int> nn(t.npad), oid(t.npad), tw(t.ntile);
+    std::vector<long long> toff(t.ntile);
+    TrajScalars ts;
+    HIPCHK(hipMemcpy(&ts, s->d.ts + traj, sizeof(ts), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nn.data(), s->d.nn + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(oid.data(), s->d.oid + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(toff.data(), s->d.tile_off + t.tbase, sizeof(long long) * t.ntile, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tw.data(), s->d.tile_w + t.tbase, sizeof(int) * t.ntile, hipMemcpyDeviceToHost));
+    long long lo = toff.empty() ? 0 : toff[0], hi = lo;
+    for (int k = 0; k < t.ntile; ++k) hi = std::max(hi, toff[k] + (long long)tw[k] * TILE);
+    std::vector<int> lst(std::max<long long>(hi - lo, 1));
+    if (hi > lo) HIPCHK(hipMemcpy(lst.data(), s->d.list + lo, sizeof(int) * (hi - lo), hipMemcpyDeviceToHost));
+    // neighbours (reference numbers, j > i) of each reference atom i
+    std::vector<std::vector<int>> nb(n);
+    for (int r = 0; r < n; ++r) {
+        const int oi = oid[r];
+        const long long base = toff[r / TILE] - lo + (r % TILE);
+        for (int q = 0; q < nn[r]; ++q) {
+            const int j = lst[base + (long long)q * TILE] - t.abase;
+            const int oj = oid[j];
+            if (oj > oi) nb[oi - 1].push_back(oj);
+        }
+    }
+    long long ind = 0;
+    for (int i = 1; i <= n; ++i) {
+        auto &v = nb[i - 1];
+        std::sort(v.begin(), v.end());
+        if (ind + 1 + (long long)v.size() > cap) return FCMD_ERR_LISTFULL;
+        atnlist[ind] = -i;
+        atnidx[i - 1] = (int)(ind + 1);
+        ++ind;
+        for (int j : v) atnlist[ind++] = j;
+    }
+    *len = ind;
+    return FCMD_OK;
+}
+
+int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series, int *fpt, int use_graph)
+{
+    if (!s || nsteps < 0) return FCMD_ERR_ARG;
+    const int chunk = cluster_every > 0 ? cluster_every : std::min(nsteps, 50);
+    int *dseries = nullptr, *dfpt = nullptr;
+    const int nsnap = cluster_every > 0 ? (nsteps + cluster_every - 1) / cluster_every : 0;
+    if (nsnap > 0) {
+        HIPCHK(hipMalloc(&dseries, sizeof(int) * (size_t)nsnap * s->ntraj));
+        HIPCHK(hipMalloc(&dfpt, sizeof(int) * s->ntraj));
+        std::vector<int> init(s->ntraj, -1);
+        if (fpt) std::copy(fpt, fpt + s->ntraj, init.begin());
+        HIPCHK(hipMemcpy(dfpt, init.data(), sizeof(int) * s->ntraj, hipMemcpyHostToDevice));
+        std::vector<ClusterScalars> cls(s->ntraj);
+        HIPCHK(hipMemcpy(cls.data(), s->d.cls, sizeof(ClusterScalars) * s->ntraj, hipMemcpyDeviceToHost));
+        for (auto &c : cls) c.want = 1;
+        HIPCHK(hipMemcpy(s->d.cls, cls.data(), sizeof(ClusterScalars) * s->ntraj, hipMemcpyHostToDevice));
+    }
+    // (re)capture a graph of `gsteps` steps when asked
+    const int gsteps = std::max(1, std::min(chunk, 10));
+    if (use_graph && (!s->graph || s->graph_steps != gsteps)) {
+        if (s->graph) {
+            (void)hipGraphExecDestroy(s->graph);
+            s->graph = nullptr;
+        }
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < gsteps; ++k) enqueue_step(s);
+        HIPCHK(hipStreamEndCapture(s->stream, &gr));
+        HIPCHK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(gr));
+        s->graph_steps = gsteps;
+    }
+    std::vector<int> step0(1);
+    {
+        TrajScalars t;
+        HIPCHK(hipMemcpy(&t, s->d.ts, sizeof(t), hipMemcpyDeviceToHost));
+        step0[0] = t.step;
+    }
+    HIPCHK(hipEventRecord(s->ev0, s->stream));
+    int done = 0, snap = 0;
+    while (done < nsteps) {
+        if (nsnap > 0 && done % cluster_every == 0) {
+            launch_cluster(s->d, s->g, s->any_fre, s->stream);
+            hipLaunchKernelGGL(k_record, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->d.ts,
+                               s->ntraj, dseries + (size_t)snap * s->ntraj, dfpt, fpt_size, step0[0] + done);
+            ++snap;
+        }
+        const int todo = std::min(nsteps - done, nsnap > 0 ? cluster_every - done % cluster_every : nsteps - done);
+        int k = 0;
+        if (use_graph)
+            for (; k + s->graph_steps <= todo; k += s->graph_steps) HIPCHK(hipGraphLaunch(s->graph, s->stream));
+        for (; k < todo; ++k) enqueue_step(s);
+        HIPCHK(hipGetLastError());
+        done += todo;
+    }
+    HIPCHK(hipEventRecord(s->ev1, s->stream));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms = ms;
+    if (nsnap > 0) {
+        if (series) HIPCHK(hipMemcpy(series, dseries, sizeof(int) * (size_t)nsnap * s->ntraj, hipMemcpyDeviceToHost));
+        if (fpt) HIPCHK(hipMemcpy(fpt, dfpt, sizeof(int) * s->ntraj, hipMemcpyDeviceToHost));
+        (void)hipFree(dseries);
+        (void)hipFree(dfpt);
+    }
+    return sync_err(s);
+}
+
+int fcmd_last_run_ms(fcmd_system *s, double *total_ms)
+{
+    if (!s || !total_ms) return FCMD_ERR_ARG;
+    *total_ms = s->last_ms;
+    return FCMD_OK;
+}
+
+int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
+{
+    if (!s || nsteps <= 0 || !avg_ms) return FCMD_ERR_ARG;
+    const int nk = 4;  // drift, rebuild sequence, force, kick
+    std::vector<hipEvent_t> ev((size_t)nsteps * (nk + 1));
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (int k = 0; k < nsteps; ++k) {
+        hipEvent_t *e = ev.data() + (size_t)k * (nk + 1);
+        HIPCHK(hipEventRecord(e[0], s->stream));
+        launch_drift(s->d, s->g.nblk, s->mixed, s->stream);
+        HIPCHK(hipEventRecord(e[1], s->stream));
+        launch_rebuild(s->d, s->g, s->mixed, s->stream);
+        HIPCHK(hipEventRecord(e[2], s->stream));
+        launch_force(s->d, s->g.nblk, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        HIPCHK(hipEventRecord(e[3], s->stream));
+        launch_kick_nh(s->d, s->g.nblk, s->mixed, s->stream);
+        HIPCHK(hipEventRecord(e[4], s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int c = 0; c < nk; ++c) avg_ms[c] = 0.0;
+    for (int k = 0; k < nsteps; ++k) {
+        hipEvent_t *e = ev.data() + (size_t)k * (nk + 1);
+        for (int c = 0; c < nk; ++c) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e[c], e[c + 1]));
+            avg_ms[c] += ms / nsteps;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return sync_err(s);
+}
+
+int fcmd_list_entries(fcmd_system *s, long long *entries)
+{
+    if (!s || !entries) return FCMD_ERR_ARG;
+    std::vector<TrajScalars> ts(s->ntraj);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(ts.data(), s->d.ts, sizeof(TrajScalars) * s->ntraj, hipMemcpyDeviceToHost));
+    long long n = 0;
+    for (auto &t : ts) n += 2LL * t.npairs;
+    *entries = n;
+    return FCMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// fcmd_common.hpp -- small device helpers shared by the kernels.
+#pragma once
+
+#include "fcmd_dev.hpp"
+
+namespace fcmd {
+
+// Fortran INT() of a double: truncation toward zero.
+__device__ __forceinline__ double fint(double v) { return (double)(int)v; }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over a BLK-thread block; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *lds4)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double *lds4)
+{
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = fmax(fmax(lds4[0], lds4[1]), fmax(lds4[2], lds4[3]));
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ int block_sum_i(int v, int *lds4)
+{
+    v = wave_sum_i(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    int r = 0;
+    if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+    __syncthreads();
+    return r;
+}
+
+// "Last block" detection: every block of a trajectory publishes its partial
+// sums, then bumps a per-trajectory counter; the block that brings it to
+// nblk runs the trajectory's scalar epilogue (and resets the counter).
+__device__ __forceinline__ bool last_block_of(unsigned int *counter, unsigned int nblk, int *lds_flag)
+{
+    __threadfence();
+    if (threadIdx.x == 0) {
+        const unsigned int prev = atomicAdd(counter, 1u);
+        *lds_flag = (prev == nblk - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = *lds_flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+// Kahan-compensated float accumulator (mixed-precision force sums).
+struct KahanF {
+    float s, c;
+    __device__ __forceinline__ KahanF() : s(0.0f), c(0.0f) {}
+    __device__ __forceinline__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+}  // namespace fcmd

@@ -1,0 +1,124 @@
+// fcmd_dev.hpp -- device-side data layout of the LJ cluster-nucleation engine.
+//
+// One allocation holds an ensemble of independent trajectories (a single
+// system is an ensemble of one).  Trajectory k owns
+//   atoms  [abase, abase + npad)   npad = natom rounded up to BLK
+//   cells  [cbase, cbase + ncpad)  enough for the smallest possible cell side
+//   tiles  [abase/TILE, ...)       one 64-atom tile per wavefront
+//   list   [lbase, lbase + lcap)   sliced-ELL neighbour list (width per tile)
+// so that every thread block belongs to exactly one trajectory and every
+// per-trajectory reduction is a fixed-order sum over its own blocks.
+//
+// Atoms are kept sorted by cell (re-sorted at each list rebuild); oid holds
+// the reference atom number (1-based, as in COMMON /ATOMC/) of each slot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fcmd {
+
+constexpr int BLK = 256;    // atoms per block (4 wavefronts of 64)
+constexpr int TILE = 64;    // atoms per neighbour-list tile (one wavefront)
+constexpr int MAXS = 10;    // species, const.inc:12 mxsorts
+constexpr int EMAX = 48;    // cluster edges (r <= atskin) kept per atom
+constexpr int NPART = 4;    // per-block partial sums
+
+// plugin variants (which reference file each entry point follows)
+enum { INT_VV = 0, INT_MAV = 1, INT_NH = 2 };     // int-verlet.f, -mav.f, -nh-mav.f
+enum { NB_PBC = 0, NB_MAV = 1 };                  // neighbor-pbc.f, neighbor-pbc-mav.f
+enum { ACC_1 = 0, ACC_M = 1, ACC_TABLE = 2 };     // for-a-lj-n-pbc.f, -m.f, explicit table
+enum { ST_N = 0, ST_NM = 1, ST_FRE = 2 };         // stoddard-pbc-n.f, -n-m.f, -n-m-fre.f
+
+enum { ERR_OK = 0, ERR_SKIN = 1, ERR_LISTFULL = 2, ERR_NH = 3, ERR_EDGES = 4, ERR_CELLS = 5 };
+
+// partial-sum slots
+enum { P_DMAX = 0, P_SUMV2 = 1, P_EPOT = 2, P_EKS = 3 };
+
+struct TrajParams {
+    double box[3], ibox[3];
+    double atrcut, atskin, atr, eps1; // cutoff, cluster radius, sigma and eps of species 1
+    double dt, Q, att, nkt;          // nkt = 3.0d0*natom*cikb*att (int-verlet-nh-mav.f:65)
+    double mas[MAXS];
+    // pair constants, index a*MAXS+b (0-based species):
+    //   c1 = sigma^6, c2 = c1*c1, c3 = 0.5*c1, e48 = 48 eps, e4 = 4 eps
+    double c1[MAXS * MAXS], c2[MAXS * MAXS], c3[MAXS * MAXS];
+    double e48[MAXS * MAXS], e4[MAXS * MAXS];
+    float c1f[MAXS * MAXS], c2f[MAXS * MAXS], c3f[MAXS * MAXS];
+    int natom, nsorts, nbtim, btim, atnenum, ethst_nh;
+    int integrator, neighbor, accatom, stoddard;
+    int abase, npad, bfirst, nblk;   // atoms, atom blocks
+    int cbase, ncpad, cfirst, ncblk; // cells, cell blocks
+    int tbase, ntile, tfirst, ntblk; // tiles, tile blocks (BLK tiles per block)
+    long long lbase, lcap;           // neighbour-list segment
+    int hbase;                       // cluster-size histograms, natom+1 entries
+    int nh_mask;                     // species bits coupled to Nose-Hoover
+    int nh_refmass;                  // 1: m1 for every coupled atom (reference)
+};
+
+// per-trajectory results of one cluster analysis
+struct ClusterScalars {
+    double cekf0;                    // cekf(0): KE of dense but not liquid-like A atoms
+    int npos;                        // Stillinger clusters with cs > 0
+    int cbig;                        // largest cs of this call
+    int nliq;                        // liquid-like atoms (= cnumf)
+    int cbigf;                       // largest tWF cluster of this call
+    int want;                        // analyse this trajectory
+    int pad;
+};
+
+struct TrajScalars {
+    double atrskin, psi, s, ekin, epot, sumv2;
+    double dmmh[256];
+    double dmms, dmmsu, dmmax, dmmav;
+    double dmmsr;                    // dmms when the last rebuild was decided
+    double rs2;                      // list radius^2 of the current list
+    double cside[3];                 // cell side used by the current list
+    int dmmst, dmmi, step, rebuild, err, npairs, nrebuild, nlist;
+    int ncx, ncy, ncz, pad0;
+    unsigned int done_a, done_b;     // last-block counters (reset by the last block)
+    int nh_iters, pad1;
+};
+
+// Device pointers of one engine.  Accelerations are double (FP64 path) or
+// float (mixed path); both sets exist, only one is used.
+struct DevPtrs {
+    double *x, *y, *z, *vx, *vy, *vz;
+    double *ax, *ay, *az;
+    float *axf, *ayf, *azf;
+    int *typ, *oid;
+    // alternate buffers for the permutation at rebuild
+    double *x2, *y2, *z2, *vx2, *vy2, *vz2, *ax2, *ay2, *az2;
+    float *axf2, *ayf2, *azf2;
+    int *typ2, *oid2;
+    int *inv;                        // inv[abase + oid - 1] = slot
+    // per block / per trajectory
+    int *blk_traj;                   // atom block -> trajectory
+    int *cblk_traj;                  // cell block -> trajectory
+    int *tblk_traj;                  // tile block -> trajectory
+    TrajParams *tp;
+    TrajScalars *ts;
+    double *part;                    // NPART per atom block
+    // cells
+    int *cell_cnt, *cell_start, *atom_cell, *atom_slot, *sorted;
+    long long *scan_tmp_ll;          // chunk totals for segmented scans
+    // neighbour list
+    int *nn, *tile_w;
+    long long *tile_off;
+    int *list;
+    // clusters: per slot
+    ClusterScalars *cls;
+    int *edeg, *edeg1, *edges, *parent, *parentf, *liq;
+    int *cs_tmp, *cgs_tmp, *csf_tmp;
+    double *cek_tmp, *cekf_tmp;
+    // clusters: per reference atom (abase + oid - 1)
+    int *liqo, *rank;
+    int *o_cl, *o_cs, *o_cgs, *o_clf, *o_ccf;
+    double *o_cek;
+    // clusters: per size (hbase + size)
+    int *o_csi, *o_csf, *o_csif, *o_cgsf, *o_cgsfa;
+    double *o_ceks, *o_cekf, *o_ceksf;
+};
+
+}  // namespace fcmd

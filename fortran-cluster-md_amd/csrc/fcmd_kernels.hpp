@@ -1,0 +1,26 @@
+// fcmd_kernels.hpp -- host-side launchers of the device kernels.
+#pragma once
+
+#include "fcmd_dev.hpp"
+
+namespace fcmd {
+
+constexpr int SCAN_CHUNK = 4 * BLK;  // cells / tiles per scan chunk
+
+enum { FORCE_STEP = 0, FORCE_ACCATOM = 1 };
+
+struct LaunchDims {
+    int nblk;    // atom blocks
+    int ncblk;   // cell chunks
+    int ntblk;   // tile chunks
+    int nhblk;   // histogram chunks
+    int ntraj;
+};
+
+void launch_drift(const DevPtrs &d, int nblk, bool mixed, hipStream_t s);
+void launch_force(const DevPtrs &d, int nblk, bool mixed, bool single_species, int mode, hipStream_t s);
+void launch_kick_nh(const DevPtrs &d, int nblk, bool mixed, hipStream_t s);
+void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
+void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s);
+
+}  // namespace fcmd

@@ -1,0 +1,309 @@
+// kernels_cluster.hip -- Stillinger and ten Wolde-Frenkel cluster analysis.
+//
+// Replaces the sequential label sweep of stoddard-pbc-n.f:69-148,
+// stoddard-pbc-n-m.f:222-307 and stoddard-pbc-n-m-fre.f:84-258.
+//
+// Edges are neighbour-list pairs with r2 <= atskin^2, r2 formed with the
+// reference's minimum image  d + L*(bf1 - int(bf2 + d/L)),  bf1 = 2 +
+// int(atrskin/sigma), bf2 = bf1 + 0.5, so the edge set is the reference's
+// bit for bit.  Components come from a lock-free union-find whose link rule
+// always hangs the root with the larger reference atom number under the
+// smaller one; the final root of a component is therefore its lowest
+// atom number, which is exactly the reference's label cl(i).  The ten
+// Wolde-Frenkel label is the 1-based rank, among liquid-like atoms in atom
+// order, of the component's lowest member (stoddard-pbc-n-m-fre.f:198-225).
+#include "fcmd_common.hpp"
+#include "fcmd_kernels.hpp"
+
+namespace fcmd {
+
+__device__ __forceinline__ int ld_relaxed(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int uf_find(int *p, int x)
+{
+    for (;;) {
+        const int px = ld_relaxed(p + x);
+        if (px == x) return x;
+        const int ppx = ld_relaxed(p + px);
+        if (ppx == px) return px;
+        atomicCAS(p + x, px, ppx);  // path halving; losing the race is harmless
+        x = ppx;
+    }
+}
+
+__device__ void uf_unite(int *p, const int *key, int a, int b)
+{
+    for (;;) {
+        a = uf_find(p, a);
+        b = uf_find(p, b);
+        if (a == b) return;
+        if (key[a] > key[b]) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(p + b, b, a) == b) return;  // b was still a root
+    }
+}
+
+__device__ __forceinline__ double kin(const DevPtrs &d, const TrajParams &tp, int i, int ty)
+{
+    const double m = (tp.stoddard == ST_N) ? tp.mas[0] : tp.mas[ty];
+    const double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
+    return 0.5 * m * (vx * vx + vy * vy + vz * vz);
+}
+
+// reset per-slot work arrays and this trajectory's histograms
+__global__ __launch_bounds__(BLK) void k_cl_init(DevPtrs d)
+{
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    if (!d.cls[t].want || d.ts[t].err) return;
+    const int i = b * BLK + threadIdx.x;
+    const int rel = i - tp.abase;
+    d.parent[i] = i;
+    d.parentf[i] = i;
+    d.cs_tmp[i] = 0;
+    d.cgs_tmp[i] = 0;
+    d.csf_tmp[i] = 0;
+    d.cek_tmp[i] = 0.0;
+    d.cekf_tmp[i] = 0.0;
+    d.liq[i] = 0;
+    if (rel < tp.natom) {
+        d.liqo[tp.abase + rel] = 0;
+        for (int k = (rel == 0 ? 0 : rel + 1); k <= rel + 1; ++k) {
+            const int h = tp.hbase + k;
+            d.o_csi[h] = 0;
+            d.o_csf[h] = 0;
+            d.o_csif[h] = 0;
+            d.o_cgsf[h] = 0;
+            d.o_cgsfa[h] = 0;
+            d.o_ceks[h] = 0.0;
+            d.o_cekf[h] = 0.0;
+            d.o_ceksf[h] = 0.0;
+        }
+    }
+    if (b == tp.bfirst && threadIdx.x == 0) {
+        ClusterScalars &c = d.cls[t];
+        c.cekf0 = 0.0;
+        c.npos = 0;
+        c.cbig = 0;
+        c.nliq = 0;
+        c.cbigf = 0;
+    }
+}
+
+// edges within atskin; degrees; liquid-like flag (tWF)
+__global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
+{
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    const TrajScalars &ts = d.ts[t];
+    if (!d.cls[t].want || ts.err) return;
+    const int i = b * BLK + threadIdx.x;
+    const int rel = i - tp.abase;
+    if (rel >= tp.natom) return;
+    const int ti = d.typ[i];
+    const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
+    const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
+    const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
+    const double bf1 = fint(ts.atrskin / tp.atr) + 2.0;
+    const double bf2 = bf1 + 0.5;
+    const double rijc2 = tp.atskin * tp.atskin;
+    const int lane = threadIdx.x & (TILE - 1);
+    const int *lp = d.list + d.tile_off[tp.tbase + rel / TILE] + lane;
+    const int nn = d.nn[i];
+    int deg = 0, deg1 = 0;
+    int *ed = d.edges + (size_t)i * EMAX;
+    const bool only_a = tp.stoddard == ST_NM;
+    if (!(only_a && ti != 0)) {
+        for (int s = 0; s < nn; ++s) {
+            const int j = lp[(size_t)s * TILE];
+            const int tj = d.typ[j];
+            if (only_a && tj != 0) continue;
+            double dx = xi - d.x[j];
+            double dy = yi - d.y[j];
+            double dz = zi - d.z[j];
+            dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
+            dy = dy + by * (bf1 - fint(bf2 + dy * iby));
+            dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= rijc2) {
+                if (deg < EMAX) ed[deg] = j;
+                ++deg;
+                deg1 += (tj == 0);
+            }
+        }
+    }
+    if (deg > EMAX) {
+        d.ts[t].err = ERR_EDGES;
+        deg = EMAX;
+    }
+    d.edeg[i] = deg;
+    d.edeg1[i] = deg1;
+    if (tp.stoddard == ST_FRE && ti == 0 && deg >= tp.atnenum) {
+        if (deg1 >= tp.atnenum) {
+            d.liq[i] = 1;
+            d.liqo[tp.abase + d.oid[i] - 1] = 1;
+            atomicAdd(&d.cls[t].nliq, 1);
+        } else {
+            atomicAdd(&d.cls[t].cekf0, kin(d, tp, i, ti));
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_cl_hook(DevPtrs d)
+{
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    if (!d.cls[t].want || d.ts[t].err) return;
+    const int i = b * BLK + threadIdx.x;
+    if (i - tp.abase >= tp.natom) return;
+    const int oi = d.oid[i];
+    const int deg = d.edeg[i];
+    const int *ed = d.edges + (size_t)i * EMAX;
+    const bool fre = tp.stoddard == ST_FRE;
+    const bool li = fre && d.liq[i];
+    for (int k = 0; k < deg; ++k) {
+        const int j = ed[k];
+        if (oi < d.oid[j]) {
+            uf_unite(d.parent, d.oid, i, j);
+            if (li && d.liq[j]) uf_unite(d.parentf, d.oid, i, j);
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_cl_count(DevPtrs d)
+{
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    if (!d.cls[t].want || d.ts[t].err) return;
+    const int i = b * BLK + threadIdx.x;
+    if (i - tp.abase >= tp.natom) return;
+    const int ti = d.typ[i];
+    const int r = uf_find(d.parent, i);
+    d.o_cl[tp.abase + d.oid[i] - 1] = d.oid[r];
+    const double ke = kin(d, tp, i, ti);
+    if (tp.stoddard == ST_FRE && ti != 0) {
+        atomicAdd(d.cgs_tmp + r, 1);
+    } else {
+        atomicAdd(d.cs_tmp + r, 1);
+        atomicAdd(d.cek_tmp + r, ke);
+    }
+    if (tp.stoddard == ST_FRE && d.liq[i]) {
+        const int rf = uf_find(d.parentf, i);
+        atomicAdd(d.csf_tmp + rf, 1);
+        atomicAdd(d.cekf_tmp + rf, ke);
+    }
+}
+
+// rank of each liquid-like atom among liquid-like atoms in atom order
+// (one block per trajectory; runs for tWF only)
+__global__ __launch_bounds__(BLK) void k_cl_rank(DevPtrs d)
+{
+    __shared__ int lds[4];
+    const int t = blockIdx.x;
+    const TrajParams &tp = d.tp[t];
+    if (!d.cls[t].want || d.ts[t].err || tp.stoddard != ST_FRE) return;
+    int carry = 0;
+    for (int b0 = 0; b0 < tp.natom; b0 += BLK) {
+        const int k = b0 + threadIdx.x;
+        const int v = k < tp.natom ? d.liqo[tp.abase + k] : 0;
+        // inclusive wave scan, then across the 4 wavefronts
+        int inc = v;
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += u;
+        }
+        const int w = threadIdx.x >> 6;
+        if (lane == 63) lds[w] = inc;
+        __syncthreads();
+        int off = carry;
+        for (int q = 0; q < w; ++q) off += lds[q];
+        const int tot = lds[0] + lds[1] + lds[2] + lds[3];
+        if (k < tp.natom) d.rank[tp.abase + k] = off + inc - v;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
+{
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    if (!d.cls[t].want || d.ts[t].err) return;
+    const int i = b * BLK + threadIdx.x;
+    if (i - tp.abase >= tp.natom) return;
+    ClusterScalars &c = d.cls[t];
+    const int oi = d.oid[i];
+    const int o = tp.abase + oi - 1;
+    const bool root = ld_relaxed(d.parent + i) == i;
+    if (root) {
+        const int sz = d.cs_tmp[i];
+        d.o_cs[o] = sz;
+        d.o_cek[o] = d.cek_tmp[i];
+        d.o_cgs[o] = d.cgs_tmp[i];
+        if (sz > 0) {
+            atomicAdd(d.o_csi + tp.hbase + sz, 1);
+            atomicAdd(d.o_ceks + tp.hbase + sz, d.cek_tmp[i]);
+            atomicAdd(&c.npos, 1);
+            atomicMax(&c.cbig, sz);
+        }
+    } else {
+        d.o_cs[o] = 0;
+        d.o_cek[o] = 0.0;
+        d.o_cgs[o] = 0;
+    }
+    if (tp.stoddard != ST_FRE) return;
+    if (d.liq[i]) {
+        const int rf = uf_find(d.parentf, i);
+        const int label = d.rank[tp.abase + d.oid[rf] - 1] + 1;
+        d.o_clf[o] = label;
+        d.o_ccf[o] = 1;
+        if (rf == i) {
+            const int sz = d.csf_tmp[i];
+            d.o_csf[tp.hbase + label] = sz;
+            d.o_cekf[tp.hbase + label] = d.cekf_tmp[i];
+            atomicAdd(d.o_csif + tp.hbase + sz, 1);
+            atomicAdd(d.o_ceksf + tp.hbase + sz, d.cekf_tmp[i]);
+            atomicMax(&c.cbigf, sz);
+        }
+    } else {
+        // last cluster to touch a non-liquid atom in the reference sweep: the
+        // largest label among its liquid-like neighbours (0 if none)
+        int lab = 0;
+        const int deg = d.edeg[i];
+        const int *ed = d.edges + (size_t)i * EMAX;
+        for (int k = 0; k < deg; ++k) {
+            const int j = ed[k];
+            if (d.liq[j]) {
+                const int rf = uf_find(d.parentf, j);
+                lab = max(lab, d.rank[tp.abase + d.oid[rf] - 1] + 1);
+            }
+        }
+        d.o_clf[o] = lab;
+        d.o_ccf[o] = 0;
+    }
+}
+
+void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cl_init, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_edges, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_hook, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_count, dim3(g.nblk), dim3(BLK), 0, s, d);
+    if (fre) hipLaunchKernelGGL(k_cl_rank, dim3(g.ntraj), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_finish, dim3(g.nblk), dim3(BLK), 0, s, d);
+}
+
+}  // namespace fcmd

@@ -1,0 +1,510 @@
+// kernels_step.hip -- one MD step: drift, LJ forces, velocity update.
+//
+// Reference routines followed (cluster/):
+//   drift      int-verlet.f:34-41, int-verlet-mav.f:107-119, int-verlet-nh-mav.f:49-61
+//   mav/skin   int-verlet-mav.f:127-137, neighbor-pbc-mav.f:178-193, neighbor-pbc.f:40-59
+//   forces     for-a-lj-n-pbc.f:54-127, for-a-lj-n-pbc-m.f:323-414
+//   kick       int-verlet.f:53-59, int-verlet-mav.f:149-155, int-verlet-nh-mav.f:91-180
+//
+// Element-wise updates use the reference's operand order and are compiled
+// without FMA contraction, so each per-atom update rounds as the reference
+// does; only sums (forces over neighbours, energies) are re-associated.
+#include "fcmd_common.hpp"
+#include "fcmd_kernels.hpp"
+
+namespace fcmd {
+
+__device__ __forceinline__ bool nh_coupled(const TrajParams &tp, int typ)
+{
+    return (tp.nh_mask >> typ) & 1;
+}
+
+// Cell grid for a list radius: side >= atrskin*(1+1e-9), so every pair within
+// atrskin lies in neighbouring cells even after rounding of x/side.
+__device__ void set_cell_grid(const TrajParams &tp, TrajScalars &ts)
+{
+    int n[3];
+    for (int k = 0; k < 3; ++k) {
+        n[k] = (int)floor(tp.box[k] / (ts.atrskin * (1.0 + 1e-9)));
+        if (n[k] < 1) n[k] = 1;
+        ts.cside[k] = tp.box[k] / (double)n[k];
+    }
+    if ((long long)n[0] * n[1] * n[2] > (long long)tp.ncpad) {
+        ts.err = ERR_CELLS;
+        return;
+    }
+    ts.ncx = n[0];
+    ts.ncy = n[1];
+    ts.ncz = n[2];
+}
+
+// Scalar part of neighbor(step): skin checks and adaptation, then flag the
+// list rebuild.  neighbor-pbc-mav.f:178-193 / neighbor-pbc.f:40-59, :96-100.
+__device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *step)
+{
+    const double rc = tp.atrcut;
+    if (tp.neighbor == NB_PBC) {
+        if (*step < 0) *step = -*step;
+        ts.dmmav = ts.dmmsu * (double)tp.nbtim / (double)(*step - tp.btim);
+        if (ts.dmms > (ts.atrskin - rc)) {
+            ts.err = ERR_SKIN;
+            return;
+        }
+        if (ts.dmmst == tp.nbtim && ts.dmms < ts.dmmav && ts.dmmav < 0.95 * (ts.atrskin - rc))
+            ts.atrskin = ts.atrskin - 0.01 * (ts.atrskin - rc);
+    } else {
+        if (ts.dmms > (ts.atrskin - rc)) {
+            ts.err = ERR_SKIN;
+            return;
+        }
+        ts.dmmav = (double)(float)ts.dmmst * ts.dmmsu / 256.0;
+        if (*step != tp.btim && *step > 0) {
+            ts.atrskin = rc + (ts.atrskin - rc) * (double)tp.nbtim / (double)ts.dmmst;
+            ts.atrskin = fmin(tp.box[0] / 2.0, ts.atrskin);
+            ts.atrskin = fmin(tp.box[1] / 2.0, ts.atrskin);
+            ts.atrskin = fmin(tp.box[2] / 2.0, ts.atrskin);
+        }
+        if (*step < 0) *step = -*step;
+    }
+    ts.rs2 = ts.atrskin * ts.atrskin;
+    set_cell_grid(tp, ts);
+    if (ts.err) return;
+    ts.dmmax = fmax(ts.dmms, ts.dmmax);
+    ts.dmmsr = ts.dmms;
+    ts.dmms = 0.0;
+    ts.dmmst = 0;
+    ts.npairs = 0;
+    ts.rebuild = 1;
+    ts.nrebuild += 1;
+}
+
+// After the drift: thermostat half update, displacement bookkeeping and the
+// rebuild decision for one trajectory (runs on one thread).
+__device__ void traj_pre(const TrajParams &tp, TrajScalars &ts, const double *part)
+{
+    double dmmr = 0.0, sumv2 = 0.0;
+    for (int b = 0; b < tp.nblk; ++b) {
+        const double *p = part + (size_t)(tp.bfirst + b) * NPART;
+        dmmr = fmax(dmmr, p[P_DMAX]);
+        sumv2 += p[P_SUMV2];
+    }
+    ts.rebuild = 0;
+    if (tp.integrator == INT_NH) {
+        ts.sumv2 = sumv2;
+        if (tp.ethst_nh) {
+            const double dt = tp.dt, dt2 = dt * dt / 2, dth = dt / 2;
+            const double m1 = tp.mas[0];
+            ts.s = ts.s + ts.psi * dt + (m1 * sumv2 - tp.nkt) * dt2 / tp.Q;
+            ts.psi = ts.psi + (m1 * sumv2 - tp.nkt) * dth / tp.Q;
+        }
+    }
+    if (tp.integrator != INT_VV) {
+        dmmr = 2.2 * sqrt(dmmr);
+        ts.dmms = ts.dmms + dmmr;
+        ts.dmmsu = ts.dmmsu + dmmr - ts.dmmh[ts.dmmi];
+        ts.dmmh[ts.dmmi] = dmmr;
+        ts.dmmi = (ts.dmmi + 1) & 255;
+        ts.dmmst = ts.dmmst + 1;
+        if (ts.dmms > (ts.atrskin - tp.atrcut - 2.0 * ts.dmms / (double)(float)ts.dmmst)) {
+            int step = ts.step;
+            neighbor_scalars(tp, ts, &step);
+        }
+    }
+}
+
+template <typename AccT>
+__device__ __forceinline__ void load_acc(const DevPtrs &d, int i, double &ax, double &ay, double &az);
+template <>
+__device__ __forceinline__ void load_acc<double>(const DevPtrs &d, int i, double &ax, double &ay, double &az)
+{
+    ax = d.ax[i];
+    ay = d.ay[i];
+    az = d.az[i];
+}
+template <>
+__device__ __forceinline__ void load_acc<float>(const DevPtrs &d, int i, double &ax, double &ay, double &az)
+{
+    ax = (double)d.axf[i];
+    ay = (double)d.ayf[i];
+    az = (double)d.azf[i];
+}
+
+template <typename AccT>
+__device__ __forceinline__ void store_acc(const DevPtrs &d, int i, double ax, double ay, double az);
+template <>
+__device__ __forceinline__ void store_acc<double>(const DevPtrs &d, int i, double ax, double ay, double az)
+{
+    d.ax[i] = ax;
+    d.ay[i] = ay;
+    d.az[i] = az;
+}
+template <>
+__device__ __forceinline__ void store_acc<float>(const DevPtrs &d, int i, double ax, double ay, double az)
+{
+    d.axf[i] = (float)ax;
+    d.ayf[i] = (float)ay;
+    d.azf[i] = (float)az;
+}
+
+// ---------------------------------------------------------------------------
+template <typename AccT>
+__global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
+{
+    __shared__ double lds[4];
+    __shared__ int flag;
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    TrajScalars &ts = d.ts[t];
+    if (ts.err) return;
+    const int i = b * BLK + threadIdx.x;
+    const bool act = (i - tp.abase) < tp.natom;
+    double d2 = 0.0, sv = 0.0;
+    if (act) {
+        const double dt = tp.dt;
+        double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
+        double ax, ay, az;
+        load_acc<AccT>(d, i, ax, ay, az);
+        double dx, dy, dz;
+        if (tp.integrator == INT_NH) {
+            const double dt2 = dt * dt / 2, dth = dt / 2;
+            const int ty = d.typ[i];
+            const bool cpl = nh_coupled(tp, ty);
+            const double psi = cpl ? ts.psi : 0.0;
+            dx = dt * vx + dt2 * (ax - psi * vx);
+            dy = dt * vy + dt2 * (ay - psi * vy);
+            dz = dt * vz + dt2 * (az - psi * vz);
+            if (cpl) {
+                // reference: m1*sum(v^2) with m1 for every atom (NH is single species)
+                sv = vx * vx + vy * vy + vz * vz;
+                if (!tp.nh_refmass) sv *= tp.mas[ty] / tp.mas[0];
+            }
+            vx = vx + dth * (ax - psi * vx);
+            vy = vy + dth * (ay - psi * vy);
+            vz = vz + dth * (az - psi * vz);
+        } else {
+            dx = (vx + 0.5 * ax * dt) * dt;
+            dy = (vy + 0.5 * ay * dt) * dt;
+            dz = (vz + 0.5 * az * dt) * dt;
+            vx = vx + 0.5 * ax * dt;
+            vy = vy + 0.5 * ay * dt;
+            vz = vz + 0.5 * az * dt;
+        }
+        d.x[i] = d.x[i] + dx;
+        d.y[i] = d.y[i] + dy;
+        d.z[i] = d.z[i] + dz;
+        d.vx[i] = vx;
+        d.vy[i] = vy;
+        d.vz[i] = vz;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    const double bmax = block_max(d2, lds);
+    const double bsum = block_sum(sv, lds);
+    if (threadIdx.x == 0) {
+        d.part[(size_t)b * NPART + P_DMAX] = bmax;
+        d.part[(size_t)b * NPART + P_SUMV2] = bsum;
+    }
+    if (last_block_of(&ts.done_a, (unsigned)tp.nblk, &flag)) {
+        if (threadIdx.x == 0) {
+            traj_pre(tp, ts, d.part);
+            ts.done_a = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Nose-Hoover implicit update, solved on one thread per trajectory.
+//
+// int-verlet-nh-mav.f:106-147 solves  v' = v_h + dth (a - psi' v'),
+// psi' = psi_h + dth (m1 sum v'^2 - 3NkT)/Q  for all 3N+1 unknowns by Newton.
+// Every solution has v' = (v_h + dth a)/(1 + dth psi'), so with
+// S = sum (v_h + dth a)^2 the system is the scalar equation
+//   g(psi) = psi_h + dth (m1 S/(1 + dth psi)^2 - 3NkT)/Q - psi = 0,
+// solved here by Newton to machine precision; the reference iteration stops
+// at a relative velocity change below 1e-10, i.e. (quadratic convergence)
+// at the same root.
+__device__ void nh_solve(const TrajParams &tp, TrajScalars &ts, double S)
+{
+    const double dth = tp.dt / 2;
+    const double A = tp.mas[0] * S;
+    const double psih = ts.psi;
+    double psi = psih;
+    int it = 0;
+    for (it = 1; it <= 100; ++it) {
+        const double den = 1.0 + dth * psi;
+        if (!(den > 0.0)) {
+            ts.err = ERR_NH;
+            return;
+        }
+        const double w = 1.0 / den;
+        const double g = psih + dth * (A * w * w - tp.nkt) / tp.Q - psi;
+        const double dg = -2.0 * dth * dth * A * w * w * w / tp.Q - 1.0;
+        const double step = -g / dg;
+        psi += step;
+        if (fabs(step) <= 1e-15 * fabs(psi) || step == 0.0) break;
+    }
+    if (it > 100) {
+        ts.err = ERR_NH;
+        return;
+    }
+    const double w = 1.0 / (1.0 + dth * psi);
+    ts.psi = psi;
+    ts.sumv2 = S * w * w;
+    ts.ekin = 0.5 * tp.mas[0] * ts.sumv2;
+    ts.nh_iters = it;
+}
+
+// Finish of AccAtom/the integrator for one trajectory (one thread).
+__device__ void traj_post(const TrajParams &tp, TrajScalars &ts, const double *part, int mode)
+{
+    double esum = 0.0, eks = 0.0;
+    for (int b = 0; b < tp.nblk; ++b) {
+        const double *p = part + (size_t)(tp.bfirst + b) * NPART;
+        esum += p[P_EPOT];
+        eks += p[P_EKS];
+    }
+    // every pair is seen from both atoms of the full list
+    esum = 0.5 * esum;
+    if (tp.accatom == ACC_1)
+        ts.epot = esum * tp.eps1 * 4.0;
+    else
+        ts.epot = esum;
+    if (mode == FORCE_ACCATOM) return;
+    if (tp.integrator == INT_NH) {
+        nh_solve(tp, ts, eks);
+    } else {
+        ts.ekin = eks;
+    }
+    ts.step += 1;
+}
+
+// ---------------------------------------------------------------------------
+// Pair sum of one atom over its full-list neighbours.  FORM = ACC_1:
+// for-a-lj-n-pbc.f:95-105 (sum of r^-2 r^-6 (c2 r^-6 - c3) d, scaled by
+// 48 eps / m afterwards); FORM = ACC_M: for-a-lj-n-pbc-m.f:377-400 (per-pair
+// 48 eps_ab r^-2 r^-6 (c2_ab r^-6 - c3_ab) d).  AccT = double: FP64
+// throughout.  AccT = float: the minimum-image difference in FP64, the pair
+// distance and LJ terms in FP32, forces summed in FP32 with Kahan
+// compensation, energies summed in FP64.
+template <typename AccT, int FORM>
+__device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
+                                         int i, int rel, int ti, double &fx, double &fy, double &fz,
+                                         double &e)
+{
+    constexpr bool MIXED = sizeof(AccT) == 4;
+    const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
+    const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
+    const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
+    const double bf1 = 2.0 + fint(ts.atrskin / tp.atr);
+    const double bf2 = bf1 + 0.5;
+    const double rc2 = tp.atrcut * tp.atrcut;
+    const int nn = d.nn[i];
+    const int lane = threadIdx.x & (TILE - 1);
+    const int tile = tp.tbase + rel / TILE;
+    const int *lp = d.list + d.tile_off[tile] + lane;
+    KahanF kx, ky, kz;
+    const float rc2f = (float)rc2;
+    fx = fy = fz = 0.0;
+    for (int s = 0; s < nn; ++s) {
+        const int j = lp[(size_t)s * TILE];
+        double dx = xi - d.x[j];
+        double dy = yi - d.y[j];
+        double dz = zi - d.z[j];
+        dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
+        dy = dy + by * (bf1 - fint(bf2 + dy * iby));
+        dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
+        const int pij = (FORM == ACC_1) ? 0 : ti * MAXS + d.typ[j];
+        if (!MIXED) {
+            double rij2 = dx * dx + dy * dy + dz * dz;
+            if (rij2 <= rc2) {
+                rij2 = 1.0 / rij2;
+                const double rij6 = rij2 * rij2 * rij2;
+                double fij;
+                if (FORM == ACC_1) {
+                    fij = rij2 * rij6 * (tp.c2[0] * rij6 - tp.c3[0]);
+                    e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
+                } else {
+                    fij = tp.e48[pij] * rij2 * rij6 * (tp.c2[pij] * rij6 - tp.c3[pij]);
+                    e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
+                }
+                fx = fx + fij * dx;
+                fy = fy + fij * dy;
+                fz = fz + fij * dz;
+            }
+        } else {
+            const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+            const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
+            if (r2 <= rc2f) {
+                const float ri2 = 1.0f / r2;
+                const float r6 = ri2 * ri2 * ri2;
+                float fij;
+                if (FORM == ACC_1) {
+                    fij = ri2 * r6 * (tp.c2f[0] * r6 - tp.c3f[0]);
+                    e += (double)(r6 * (tp.c2f[0] * r6 - tp.c1f[0]));
+                } else {
+                    fij = (float)tp.e48[pij] * ri2 * r6 * (tp.c2f[pij] * r6 - tp.c3f[pij]);
+                    e += tp.e4[pij] * (double)(r6 * (tp.c2f[pij] * r6 - tp.c1f[pij]));
+                }
+                kx.add(fij * dxf);
+                ky.add(fij * dyf);
+                kz.add(fij * dzf);
+            }
+        }
+    }
+    if (MIXED) {
+        fx = (double)kx.s;
+        fy = (double)ky.s;
+        fz = (double)kz.s;
+    }
+}
+
+// LJ forces over the full neighbour list plus the integrator's fused
+// epilogue.  MULTI = false: every trajectory of the launch is single-species
+// (ACC_1); MULTI = true: each trajectory uses its own form.
+template <typename AccT, bool MULTI>
+__global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
+{
+    constexpr bool MIXED = sizeof(AccT) == 4;
+    __shared__ double lds[4];
+    __shared__ int flag;
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    TrajScalars &ts = d.ts[t];
+    if (ts.err) return;
+    const int i = b * BLK + threadIdx.x;
+    const int rel = i - tp.abase;
+    const bool act = rel < tp.natom;
+    const bool form1 = !MULTI || tp.accatom == ACC_1;
+    double e = 0.0, eks = 0.0;
+    if (act) {
+        const int ti = d.typ[i];
+        double fx, fy, fz;
+        if (form1)
+            pair_sum<AccT, ACC_1>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+        else
+            pair_sum<AccT, ACC_M>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+        // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:408-414
+        double ax, ay, az;
+        double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
+        if (mode == FORCE_ACCATOM) load_acc<AccT>(d, i, ax0, ay0, az0);
+        if (form1) {
+            const double iatm = 1.0 / tp.mas[0];
+            ax = ax0 + tp.e48[0] * fx * iatm;
+            ay = ay0 + tp.e48[0] * fy * iatm;
+            az = az0 + tp.e48[0] * fz * iatm;
+        } else {
+            const double iatm = 1.0 / tp.mas[ti];
+            ax = ax0 + fx * iatm;
+            ay = ay0 + fy * iatm;
+            az = az0 + fz * iatm;
+        }
+        store_acc<AccT>(d, i, ax, ay, az);
+        if (MIXED) load_acc<AccT>(d, i, ax, ay, az);  // integrate what is stored
+        if (mode == FORCE_STEP) {
+            const double dt = tp.dt;
+            double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
+            if (tp.integrator == INT_NH) {
+                if (nh_coupled(tp, ti)) {
+                    const double dth = dt / 2;
+                    const double wx = vx + dth * ax, wy = vy + dth * ay, wz = vz + dth * az;
+                    eks = wx * wx + wy * wy + wz * wz;
+                    if (!tp.nh_refmass) eks *= tp.mas[ti] / tp.mas[0];
+                }
+            } else {
+                vx = vx + 0.5 * ax * dt;
+                vy = vy + 0.5 * ay * dt;
+                vz = vz + 0.5 * az * dt;
+                d.vx[i] = vx;
+                d.vy[i] = vy;
+                d.vz[i] = vz;
+                const double m = (tp.integrator == INT_VV) ? tp.mas[0] : tp.mas[ti];
+                eks = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+            }
+        }
+    }
+    const double be = block_sum(e, lds);
+    const double bk = block_sum(eks, lds);
+    if (threadIdx.x == 0) {
+        d.part[(size_t)b * NPART + P_EPOT] = be;
+        d.part[(size_t)b * NPART + P_EKS] = bk;
+    }
+    if (last_block_of(&ts.done_b, (unsigned)tp.nblk, &flag)) {
+        if (threadIdx.x == 0) {
+            traj_post(tp, ts, d.part, mode);
+            ts.done_b = 0;
+        }
+    }
+}
+
+// Nose-Hoover final velocities v' = (v_h + dth a)/(1 + dth psi').
+template <typename AccT>
+__global__ __launch_bounds__(BLK) void k_kick_nh(DevPtrs d)
+{
+    const int b = blockIdx.x;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    const TrajScalars &ts = d.ts[t];
+    if (ts.err || tp.integrator != INT_NH) return;
+    const int i = b * BLK + threadIdx.x;
+    if (i - tp.abase >= tp.natom) return;
+    const double dth = tp.dt / 2;
+    double ax, ay, az;
+    load_acc<AccT>(d, i, ax, ay, az);
+    const double den = nh_coupled(tp, d.typ[i]) ? 1.0 + dth * ts.psi : 1.0;
+    d.vx[i] = (d.vx[i] + dth * ax) / den;
+    d.vy[i] = (d.vy[i] + dth * ay) / den;
+    d.vz[i] = (d.vz[i] + dth * az) / den;
+}
+
+// neighbor(step) called directly (init-cube1.f:113, read-rst.f:63): only
+// trajectory `traj` rebuilds
+__global__ void k_neighbor_entry(DevPtrs d, int traj, int ntraj, int step, int *step_out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int u = 0; u < ntraj; ++u) d.ts[u].rebuild = 0;
+    TrajScalars &ts = d.ts[traj];
+    if (ts.err) return;
+    int s = step;
+    neighbor_scalars(d.tp[traj], ts, &s);
+    *step_out = s;
+}
+
+// ---------------------------------------------------------------------------
+void launch_neighbor_entry(const DevPtrs &d, int traj, int ntraj, int step, int *step_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neighbor_entry, dim3(1), dim3(64), 0, s, d, traj, ntraj, step, step_out);
+}
+
+void launch_drift(const DevPtrs &d, int nblk, bool mixed, hipStream_t s)
+{
+    if (mixed)
+        hipLaunchKernelGGL(k_drift<float>, dim3(nblk), dim3(BLK), 0, s, d);
+    else
+        hipLaunchKernelGGL(k_drift<double>, dim3(nblk), dim3(BLK), 0, s, d);
+}
+
+void launch_force(const DevPtrs &d, int nblk, bool mixed, bool single_species, int mode, hipStream_t s)
+{
+    if (mixed) {
+        if (single_species)
+            hipLaunchKernelGGL((k_force<float, false>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+        else
+            hipLaunchKernelGGL((k_force<float, true>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+    } else {
+        if (single_species)
+            hipLaunchKernelGGL((k_force<double, false>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+        else
+            hipLaunchKernelGGL((k_force<double, true>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+    }
+}
+
+void launch_kick_nh(const DevPtrs &d, int nblk, bool mixed, hipStream_t s)
+{
+    if (mixed)
+        hipLaunchKernelGGL(k_kick_nh<float>, dim3(nblk), dim3(BLK), 0, s, d);
+    else
+        hipLaunchKernelGGL(k_kick_nh<double>, dim3(nblk), dim3(BLK), 0, s, d);
+}
+
+}  // namespace fcmd

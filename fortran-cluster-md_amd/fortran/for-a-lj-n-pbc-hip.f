@@ -1,0 +1,32 @@
+*===============================================================================
+*	ACCELERATION ATOM-ATOM (LJ, one species) ON THE GPU
+*===============================================================================
+* file: for-a-lj-n-pbc-hip.f
+*-------------------------------------------------------------------------------
+* Drop-in for for-a-lj-n-pbc.f:14-129: same subroutine name and
+* arguments, computed on the GPU through the C ABI (include/fcmd.h).
+*-------------------------------------------------------------------------------
+
+	subroutine AccAtom()
+
+	use fcmd_dropin_state
+	implicit none
+
+	integer(c_int) :: rc
+
+	call fcmdensure()
+	call fcmdpushdirty()
+	rc = fcmd_accatom(fcmd_sys)
+	call fcmd_stop_on_error(rc)
+	call fcmdpull()
+	return
+	end
+
+	subroutine fcmdvaracc(v)
+	use fcmd_capi
+	implicit none
+	integer(c_int) :: v
+	v = FCMD_ACC_1
+	return
+	end
+*-------------------------------------------------------------------------------

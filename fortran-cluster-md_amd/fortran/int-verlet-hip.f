@@ -1,0 +1,34 @@
+*===============================================================================
+*	INTEGRATOR VERLET ON THE GPU
+*===============================================================================
+* file: int-verlet-hip.f
+*-------------------------------------------------------------------------------
+* Drop-in for int-verlet.f:16-62: same subroutine name and
+* arguments, computed on the GPU through the C ABI (include/fcmd.h).
+*-------------------------------------------------------------------------------
+
+	subroutine integrator(step)
+
+	use fcmd_dropin_state
+	implicit none
+
+	integer step
+	integer(c_int) :: st, rc
+
+	call fcmdensure()
+	call fcmdpushdirty()
+	st = step
+	rc = fcmd_integrator(fcmd_sys, st)
+	call fcmd_stop_on_error(rc)
+	call fcmdafter(step)
+	return
+	end
+
+	subroutine fcmdvarint(v)
+	use fcmd_capi
+	implicit none
+	integer(c_int) :: v
+	v = FCMD_INT_VV
+	return
+	end
+*-------------------------------------------------------------------------------

@@ -1,0 +1,51 @@
+*===============================================================================
+*	DETERMINATION OF CLUSTER SIZES (species 1) ON THE GPU
+*===============================================================================
+* file: stoddard-pbc-n-m-hip.f
+*-------------------------------------------------------------------------------
+* Drop-in for stoddard-pbc-n-m.f:163-310: same subroutine name and
+* arguments, computed on the GPU through the C ABI (include/fcmd.h).
+*-------------------------------------------------------------------------------
+
+	subroutine stoddard(step)
+
+	use fcmd_dropin_state
+	implicit none
+
+	include 'const.inc'
+	include 'atomp.inc'
+	include 'cvtpp.inc'
+
+	integer step, i
+	integer(c_int) :: rc, cnt(6)
+
+	call fcmdensure()
+	call fcmdpushdirty()
+	cnt(1) = cnum
+	cnt(2) = cnumf
+	cnt(3) = cbiggest
+	cnt(4) = cbiggestf
+	cnt(5) = clookfor
+	cnt(6) = clookforf
+	rc = fcmd_stoddard(fcmd_sys, 0, step, cl, cs, cc, csi, cek, ceks,
+     $	       clf, csf, ccf, csif, cekf, ceksf, cgs, cgsf, cgsfa,
+     $	       cfpt, cfptf, cnt)
+	call fcmd_stop_on_error(rc)
+	cstep = step
+	cnum = cnt(1)
+	cnumf = cnt(2)
+	cbiggest = cnt(3)
+	cbiggestf = cnt(4)
+	clookfor = cnt(5)
+	clookforf = cnt(6)
+	return
+	end
+
+	subroutine fcmdvarst(v)
+	use fcmd_capi
+	implicit none
+	integer(c_int) :: v
+	v = FCMD_ST_NM
+	return
+	end
+*-------------------------------------------------------------------------------

@@ -1,0 +1,173 @@
+/*
+ * fcmd.h -- C ABI of the MI355X LJ cluster-nucleation engine.
+ *
+ * Drop-in boundary for the reference's plugin subroutines (janw76/
+ * fortran-cluster-md, cluster/), which are external F77 subroutines linked
+ * by name and sharing state through COMMON blocks (Makefile:8-114):
+ *
+ *   AccAtom()          for-a-lj-n-pbc.f:14, for-a-lj-n-pbc-m.f:280  -> fcmd_accatom
+ *   neighbor(step)     neighbor-pbc.f:11, neighbor-pbc-mav.f:143    -> fcmd_neighbor
+ *   integrator(step)   int-verlet.f:16, int-verlet-mav.f:81,
+ *                      int-verlet-nh-mav.f:13                       -> fcmd_integrator
+ *   stoddard(step)     stoddard-pbc-n.f:10, stoddard-pbc-n-m.f:163,
+ *                      stoddard-pbc-n-m-fre.f:11                    -> fcmd_stoddard
+ *
+ * The Fortran shims in fortran-cluster-md_amd/fortran/ keep those names and
+ * argument lists and call these functions through ISO_C_BINDING.  All arrays
+ * are passed in the reference's order (atom i of COMMON /ATOMC/ is element
+ * i-1); units are the reference's internal units (Angstrom, fs, 1e-27 kg,
+ * 1e-17 J, K).  State stays resident on the GPU between calls; fcmd_upload /
+ * fcmd_download move it.  Every function returns 0 or an FCMD_ERR_* code;
+ * fcmd_error_text gives the reference's message for it.
+ *
+ * One fcmd_system holds one or more independent trajectories (an ensemble);
+ * the drop-in uses one.  Calls on one fcmd_system must not overlap.
+ */
+#ifndef FCMD_H
+#define FCMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCMD_MAXSORTS 10
+
+enum { FCMD_FP64 = 0, FCMD_MIXED = 1 };
+enum { FCMD_INT_VV = 0, FCMD_INT_MAV = 1, FCMD_INT_NH = 2 };
+enum { FCMD_NB_PBC = 0, FCMD_NB_MAV = 1 };
+enum { FCMD_ACC_1 = 0, FCMD_ACC_M = 1, FCMD_ACC_TABLE = 2 };
+enum { FCMD_ST_N = 0, FCMD_ST_NM = 1, FCMD_ST_FRE = 2 };
+
+enum {
+    FCMD_OK = 0,
+    FCMD_ERR_SKIN = 1,      /* neighbor-pbc*.f: "Skin radius exceeded!!" */
+    FCMD_ERR_LISTFULL = 2,  /* neighbor-pbc*.f: "Neighborlist is nearly full!" */
+    FCMD_ERR_NH = 3,        /* int-verlet-nh-mav.f:151 "did not converge" */
+    FCMD_ERR_EDGES = 4,     /* more cluster edges per atom than the engine keeps */
+    FCMD_ERR_CELLS = 5,     /* cell grid larger than allocated */
+    FCMD_ERR_ARG = 10,
+    FCMD_ERR_HIP = 11,
+    FCMD_ERR_NODEVICE = 12
+};
+
+typedef struct fcmd_system fcmd_system;
+
+/* Parameters of one trajectory: the COMMON values the hot path reads. */
+typedef struct {
+    int natom, nsorts;                     /* ATOMP natom, atsorts */
+    double box[3];                         /* BOXPP boxx, boxy, boxz */
+    double mas[FCMD_MAXSORTS];             /* apmas */
+    double sig[FCMD_MAXSORTS];             /* apsig */
+    double eps[FCMD_MAXSORTS];             /* apeps */
+    int num[FCMD_MAXSORTS];                /* apnum */
+    double atrcut, atrskin, atskin;        /* in length units (read-3d.f:259-261) */
+    double dt, Q, att;                     /* TIMEP dt; ATOMP Q (internal), att */
+    int nbtim, btim, atnenum, ethst_nh;    /* ethst_nh: ENERG ethst == 'N' */
+    int integrator, neighbor, accatom, stoddard;   /* which reference file */
+    double eps_ab[FCMD_MAXSORTS * FCMD_MAXSORTS];  /* FCMD_ACC_TABLE only */
+    double sig_ab[FCMD_MAXSORTS * FCMD_MAXSORTS];
+    int nh_mask;    /* species bits coupled to Nose-Hoover; 0 = every atom with m1 (reference) */
+    int pad;
+} fcmd_params;
+
+/* Scalars of one trajectory (ATOMP atrskin, psi, s; ENERG ekin, epot,
+ * sumv2; DMMPP), plus engine counters: dmmsr is dmms as it stood when the
+ * last rebuild was decided (what neighbor-pbc-mav.f:232 prints), npairs the
+ * pairs in the current list, nrebuild the rebuilds so far. */
+typedef struct {
+    double atrskin, psi, s, ekin, epot, sumv2;
+    double dmmh[256];
+    double dmms, dmmsu, dmmax, dmmav, dmmsr;
+    int dmmst, dmmi, step, npairs, nrebuild, nh_iters, err, nlist;
+} fcmd_scalars;
+
+int fcmd_device_count(void);
+const char *fcmd_error_text(int code);
+
+/* Allocate an ensemble of ntraj trajectories on the current HIP device.
+ * precision: FCMD_FP64 or FCMD_MIXED.  list_width: neighbour-list slots per
+ * atom reserved (the role of const.inc mxnlist); 0 selects 128. */
+int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int precision,
+                int list_width);
+void fcmd_destroy(fcmd_system *s);
+
+/* Replace the parameters of trajectory traj (same natom, box within the
+ * allocated cell grid). */
+int fcmd_set_params(fcmd_system *s, int traj, const fcmd_params *p);
+
+/* ATOMC x,v,a and ATOMP api (species 1..) of one trajectory, reference
+ * order.  Pointers may be NULL to leave that array as it is. */
+int fcmd_upload(fcmd_system *s, int traj, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *ax,
+                const double *ay, const double *az, const int *api);
+int fcmd_download(fcmd_system *s, int traj, double *x, double *y, double *z, double *vx,
+                  double *vy, double *vz, double *ax, double *ay, double *az);
+
+int fcmd_set_scalars(fcmd_system *s, int traj, const fcmd_scalars *sc);
+int fcmd_get_scalars(fcmd_system *s, int traj, fcmd_scalars *sc);
+
+/* neighbor(step): neighbor-pbc.f:11 / neighbor-pbc-mav.f:143.  Wraps the
+ * positions, applies the skin rules and rebuilds the list of trajectory traj;
+ * *step is made non-negative as the reference does. */
+int fcmd_neighbor(fcmd_system *s, int traj, int *step);
+
+/* AccAtom(): for-a-lj-n-pbc.f:14 / for-a-lj-n-pbc-m.f:280.  Adds the LJ
+ * accelerations to a and sets epot, for every trajectory. */
+int fcmd_accatom(fcmd_system *s);
+
+/* integrator(step): one step of every trajectory (int-verlet.f:16,
+ * int-verlet-mav.f:81, int-verlet-nh-mav.f:13), including the list rebuild
+ * the displacement rule asks for.  *step is this step's number. */
+int fcmd_integrator(fcmd_system *s, int *step);
+
+/* stoddard(step): stoddard-pbc-n.f:10 / -n-m.f:163 / -n-m-fre.f:11 for
+ * trajectory traj.  Arrays in the CVTPP layout; histogram arrays hold
+ * natom+1 entries starting at index 0.  csi[0] and ceks[0] are updated the
+ * way the reference leaves them (csi(0) is never reset there).  counters =
+ * {cnum, cnumf, cbiggest, cbiggestf, clookfor, clookforf} in/out.  tWF
+ * arrays may be NULL unless the trajectory uses FCMD_ST_FRE. */
+int fcmd_stoddard(fcmd_system *s, int traj, int step, int *cl, int *cs, int *cc, int *csi,
+                  double *cek, double *ceks, int *clf, int *csf, int *ccf, int *csif,
+                  double *cekf, double *ceksf, int *cgs, int *cgsf, int *cgsfa, int *cfpt,
+                  int *cfptf, int *counters);
+
+/* The current list of trajectory traj in the reference's ATOMP layout:
+ * atnlist = -1, j.., -2, j.., ..., -natom (ascending j > i) and atnidx.
+ * *len receives the number of entries written (<= cap). */
+int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *atnidx,
+                  long long *len);
+
+/* Ensemble run: nsteps integrator steps of every trajectory with no host
+ * decision inside (HIP-graph replay when use_graph), a Stillinger analysis
+ * every cluster_every steps (0: none) recording each trajectory's largest
+ * cluster in series[snapshot*ntraj + traj] (may be NULL) and the first step
+ * at which it reached fpt_size in fpt[traj] (-1 if never; may be NULL).
+ * The analysis at step i uses the state before step i is integrated, as the
+ * reference's main loop (main.f:133-164). */
+int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series,
+             int *fpt, int use_graph);
+
+/* Seeded initial state: natom positions uniform in the box with no pair
+ * closer than rmin (minimum image), Maxwell-Boltzmann velocities at
+ * temperature att for masses mas[api-1], zero total momentum, rescaled to
+ * exactly att (3 N k T / 2 kinetic energy, k = 1.38066e-6).  Deterministic in
+ * seed.  Host only. */
+int fcmd_init_random(unsigned long long seed, int natom, const double *box, double rmin,
+                     double att, const double *mas, const int *api, double *x, double *y,
+                     double *z, double *vx, double *vy, double *vz);
+
+/* Time of the last fcmd_run in milliseconds (HIP events on the run stream). */
+int fcmd_last_run_ms(fcmd_system *s, double *total_ms);
+
+/* Run nsteps steps (no graph) with HIP events on the engine stream around
+ * each kernel class; avg_ms[4] receives the mean milliseconds per step of
+ * {drift, list-rebuild sequence, force, Nose-Hoover kick}. */
+int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms);
+
+/* Full-list entries (2 x pairs) summed over all trajectories' current lists. */
+int fcmd_list_entries(fcmd_system *s, long long *entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
