@@ -141,7 +141,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step's kernels on the stream instead of replaying a graph")
     args = ap.parse_args()
+    use_graph = not args.no_graph
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "trajectories shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
@@ -161,10 +164,10 @@ def main():
     for name, prec in (("mixed", fcmd.MIXED), ("fp64", fcmd.FP64)):
         log(f"[bench] {name}: setup {len(params)} trajectories, {natoms} atoms")
         s = build_system(params, states, prec)
-        s.run(Wm, cluster_every=0, use_graph=True)
+        s.run(Wm, cluster_every=0, use_graph=use_graph)
         h0 = hamiltonian(s, params)
         t0 = time.perf_counter()
-        series, fpt = s.run(K, cluster_every=every, fpt_size=fpt_size, use_graph=True)
+        series, fpt = s.run(K, cluster_every=every, fpt_size=fpt_size, use_graph=use_graph)
         wall = time.perf_counter() - t0
         h1 = hamiltonian(s, params)
         ev_ms = s.last_run_ms()

@@ -232,9 +232,9 @@ int sync_err(fcmd_system *s)
 // one integrator step of every trajectory
 void enqueue_step(fcmd_system *s)
 {
-    launch_drift(s->d, s->g.nblk, s->mixed, s->stream);
+    launch_drift(s->d, s->g, s->mixed, s->stream);
     launch_rebuild(s->d, s->g, s->mixed, s->stream);
-    launch_force(s->d, s->g.nblk, s->mixed, s->single_species, FORCE_STEP, s->stream);
+    launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
     launch_kick_nh(s->d, s->g.nblk, s->mixed, s->stream);
 }
 
@@ -547,7 +547,7 @@ int fcmd_neighbor(fcmd_system *s, int traj, int *step)
 int fcmd_accatom(fcmd_system *s)
 {
     if (!s) return FCMD_ERR_ARG;
-    launch_force(s->d, s->g.nblk, s->mixed, s->single_species, FORCE_ACCATOM, s->stream);
+    launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_ACCATOM, s->stream);
     HIPCHK(hipGetLastError());
     return sync_err(s);
 }
@@ -765,11 +765,11 @@ int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
     for (int k = 0; k < nsteps; ++k) {
         hipEvent_t *e = ev.data() + (size_t)k * (nk + 1);
         HIPCHK(hipEventRecord(e[0], s->stream));
-        launch_drift(s->d, s->g.nblk, s->mixed, s->stream);
+        launch_drift(s->d, s->g, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[1], s->stream));
         launch_rebuild(s->d, s->g, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[2], s->stream));
-        launch_force(s->d, s->g.nblk, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
         HIPCHK(hipEventRecord(e[3], s->stream));
         launch_kick_nh(s->d, s->g.nblk, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[4], s->stream));

@@ -73,22 +73,6 @@ __device__ __forceinline__ int block_sum_i(int v, int *lds4)
     return r;
 }
 
-// "Last block" detection: every block of a trajectory publishes its partial
-// sums, then bumps a per-trajectory counter; the block that brings it to
-// nblk runs the trajectory's scalar epilogue (and resets the counter).
-__device__ __forceinline__ bool last_block_of(unsigned int *counter, unsigned int nblk, int *lds_flag)
-{
-    __threadfence();
-    if (threadIdx.x == 0) {
-        const unsigned int prev = atomicAdd(counter, 1u);
-        *lds_flag = (prev == nblk - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = *lds_flag != 0;
-    if (last) __threadfence();
-    return last;
-}
-
 // Kahan-compensated float accumulator (mixed-precision force sums).
 struct KahanF {
     float s, c;

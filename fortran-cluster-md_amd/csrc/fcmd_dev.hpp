@@ -77,7 +77,6 @@ struct TrajScalars {
     double cside[3];                 // cell side used by the current list
     int dmmst, dmmi, step, rebuild, err, npairs, nrebuild, nlist;
     int ncx, ncy, ncz, pad0;
-    unsigned int done_a, done_b;     // last-block counters (reset by the last block)
     int nh_iters, pad1;
 };
 

@@ -17,8 +17,9 @@ struct LaunchDims {
     int ntraj;
 };
 
-void launch_drift(const DevPtrs &d, int nblk, bool mixed, hipStream_t s);
-void launch_force(const DevPtrs &d, int nblk, bool mixed, bool single_species, int mode, hipStream_t s);
+void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
+void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
+                  hipStream_t s);
 void launch_kick_nh(const DevPtrs &d, int nblk, bool mixed, hipStream_t s);
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s);

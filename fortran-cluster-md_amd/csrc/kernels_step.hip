@@ -78,16 +78,28 @@ __device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *ste
     ts.nrebuild += 1;
 }
 
+// Fixed-order reduction of one trajectory's per-block partials by one block
+// (deterministic: thread k takes blocks k, k+BLK, ...; then a fixed tree).
+// Returns in thread 0: sum of slot s0, sum of slot s1, max or sum of slot s2.
+__device__ void reduce_partials(const TrajParams &tp, const double *part, int s0, int s1, int s2,
+                                bool max2, double *lds, double &r0, double &r1, double &r2)
+{
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < tp.nblk; k += BLK) {
+        const double *p = part + (size_t)(tp.bfirst + k) * NPART;
+        a += p[s0];
+        b += p[s1];
+        c = max2 ? fmax(c, p[s2]) : c + p[s2];
+    }
+    r0 = block_sum(a, lds);
+    r1 = block_sum(b, lds);
+    r2 = max2 ? block_max(c, lds) : block_sum(c, lds);
+}
+
 // After the drift: thermostat half update, displacement bookkeeping and the
 // rebuild decision for one trajectory (runs on one thread).
-__device__ void traj_pre(const TrajParams &tp, TrajScalars &ts, const double *part)
+__device__ void traj_pre(const TrajParams &tp, TrajScalars &ts, double dmmr, double sumv2)
 {
-    double dmmr = 0.0, sumv2 = 0.0;
-    for (int b = 0; b < tp.nblk; ++b) {
-        const double *p = part + (size_t)(tp.bfirst + b) * NPART;
-        dmmr = fmax(dmmr, p[P_DMAX]);
-        sumv2 += p[P_SUMV2];
-    }
     ts.rebuild = 0;
     if (tp.integrator == INT_NH) {
         ts.sumv2 = sumv2;
@@ -151,7 +163,6 @@ template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
 {
     __shared__ double lds[4];
-    __shared__ int flag;
     const int b = blockIdx.x;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
@@ -204,12 +215,19 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
         d.part[(size_t)b * NPART + P_DMAX] = bmax;
         d.part[(size_t)b * NPART + P_SUMV2] = bsum;
     }
-    if (last_block_of(&ts.done_a, (unsigned)tp.nblk, &flag)) {
-        if (threadIdx.x == 0) {
-            traj_pre(tp, ts, d.part);
-            ts.done_a = 0;
-        }
-    }
+}
+
+// one block per trajectory, after the drift
+__global__ __launch_bounds__(BLK) void k_traj_pre(DevPtrs d)
+{
+    __shared__ double lds[4];
+    const int t = blockIdx.x;
+    const TrajParams &tp = d.tp[t];
+    TrajScalars &ts = d.ts[t];
+    if (ts.err) return;
+    double unused, sumv2, dmmr;
+    reduce_partials(tp, d.part, P_EPOT, P_SUMV2, P_DMAX, true, lds, unused, sumv2, dmmr);
+    if (threadIdx.x == 0) traj_pre(tp, ts, dmmr, sumv2);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,6 +246,9 @@ __device__ void nh_solve(const TrajParams &tp, TrajScalars &ts, double S)
     const double dth = tp.dt / 2;
     const double A = tp.mas[0] * S;
     const double psih = ts.psi;
+    // g is a difference of terms of this size; its rounding noise (and so
+    // the smallest meaningful Newton step) is a few ulps of it
+    const double gscale = fabs(psih) + dth * (A + tp.nkt) / tp.Q;
     double psi = psih;
     int it = 0;
     for (it = 1; it <= 100; ++it) {
@@ -241,7 +262,7 @@ __device__ void nh_solve(const TrajParams &tp, TrajScalars &ts, double S)
         const double dg = -2.0 * dth * dth * A * w * w * w / tp.Q - 1.0;
         const double step = -g / dg;
         psi += step;
-        if (fabs(step) <= 1e-15 * fabs(psi) || step == 0.0) break;
+        if (fabs(step) <= 8e-16 * (fabs(psi) + gscale)) break;
     }
     if (it > 100) {
         ts.err = ERR_NH;
@@ -254,15 +275,13 @@ __device__ void nh_solve(const TrajParams &tp, TrajScalars &ts, double S)
     ts.nh_iters = it;
 }
 
-// Finish of AccAtom/the integrator for one trajectory (one thread).
-__device__ void traj_post(const TrajParams &tp, TrajScalars &ts, const double *part, int mode)
+// Finish of AccAtom/the integrator for one trajectory (one thread).  esum:
+// per-atom pair energies over the full list; eks: VV kinetic energy, or the
+// Nose-Hoover S; ekfree: kinetic energy of atoms not coupled to the
+// thermostat (species-selective coupling only).
+__device__ void traj_post(const TrajParams &tp, TrajScalars &ts, double esum, double eks, double ekfree,
+                          int mode)
 {
-    double esum = 0.0, eks = 0.0;
-    for (int b = 0; b < tp.nblk; ++b) {
-        const double *p = part + (size_t)(tp.bfirst + b) * NPART;
-        esum += p[P_EPOT];
-        eks += p[P_EKS];
-    }
     // every pair is seen from both atoms of the full list
     esum = 0.5 * esum;
     if (tp.accatom == ACC_1)
@@ -272,10 +291,24 @@ __device__ void traj_post(const TrajParams &tp, TrajScalars &ts, const double *p
     if (mode == FORCE_ACCATOM) return;
     if (tp.integrator == INT_NH) {
         nh_solve(tp, ts, eks);
+        ts.ekin += ekfree;
     } else {
         ts.ekin = eks;
     }
     ts.step += 1;
+}
+
+// one block per trajectory, after the forces
+__global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
+{
+    __shared__ double lds[4];
+    const int t = blockIdx.x;
+    const TrajParams &tp = d.tp[t];
+    TrajScalars &ts = d.ts[t];
+    if (ts.err) return;
+    double esum, eks, ekfree;
+    reduce_partials(tp, d.part, P_EPOT, P_EKS, P_SUMV2, false, lds, esum, eks, ekfree);
+    if (threadIdx.x == 0) traj_post(tp, ts, esum, eks, ekfree, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -366,17 +399,16 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
     __shared__ double lds[4];
-    __shared__ int flag;
     const int b = blockIdx.x;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
-    TrajScalars &ts = d.ts[t];
+    const TrajScalars &ts = d.ts[t];
     if (ts.err) return;
     const int i = b * BLK + threadIdx.x;
     const int rel = i - tp.abase;
     const bool act = rel < tp.natom;
     const bool form1 = !MULTI || tp.accatom == ACC_1;
-    double e = 0.0, eks = 0.0;
+    double e = 0.0, eks = 0.0, ekfree = 0.0;
     if (act) {
         const int ti = d.typ[i];
         double fx, fy, fz;
@@ -405,11 +437,14 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
             const double dt = tp.dt;
             double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
             if (tp.integrator == INT_NH) {
+                const double dth = dt / 2;
+                const double wx = vx + dth * ax, wy = vy + dth * ay, wz = vz + dth * az;
                 if (nh_coupled(tp, ti)) {
-                    const double dth = dt / 2;
-                    const double wx = vx + dth * ax, wy = vy + dth * ay, wz = vz + dth * az;
                     eks = wx * wx + wy * wy + wz * wz;
                     if (!tp.nh_refmass) eks *= tp.mas[ti] / tp.mas[0];
+                } else {
+                    // not thermostatted: its final velocity is already known
+                    ekfree = 0.5 * tp.mas[ti] * (wx * wx + wy * wy + wz * wz);
                 }
             } else {
                 vx = vx + 0.5 * ax * dt;
@@ -425,15 +460,11 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
     }
     const double be = block_sum(e, lds);
     const double bk = block_sum(eks, lds);
+    const double bf = block_sum(ekfree, lds);
     if (threadIdx.x == 0) {
         d.part[(size_t)b * NPART + P_EPOT] = be;
         d.part[(size_t)b * NPART + P_EKS] = bk;
-    }
-    if (last_block_of(&ts.done_b, (unsigned)tp.nblk, &flag)) {
-        if (threadIdx.x == 0) {
-            traj_post(tp, ts, d.part, mode);
-            ts.done_b = 0;
-        }
+        d.part[(size_t)b * NPART + P_SUMV2] = bf;
     }
 }
 
@@ -476,27 +507,31 @@ void launch_neighbor_entry(const DevPtrs &d, int traj, int ntraj, int step, int 
     hipLaunchKernelGGL(k_neighbor_entry, dim3(1), dim3(64), 0, s, d, traj, ntraj, step, step_out);
 }
 
-void launch_drift(const DevPtrs &d, int nblk, bool mixed, hipStream_t s)
+void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
     if (mixed)
-        hipLaunchKernelGGL(k_drift<float>, dim3(nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_drift<float>, dim3(g.nblk), dim3(BLK), 0, s, d);
     else
-        hipLaunchKernelGGL(k_drift<double>, dim3(nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_drift<double>, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_traj_pre, dim3(g.ntraj), dim3(BLK), 0, s, d);
 }
 
-void launch_force(const DevPtrs &d, int nblk, bool mixed, bool single_species, int mode, hipStream_t s)
+void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
+                  hipStream_t s)
 {
+    const dim3 grid(g.nblk), blk(BLK);
     if (mixed) {
         if (single_species)
-            hipLaunchKernelGGL((k_force<float, false>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+            hipLaunchKernelGGL((k_force<float, false>), grid, blk, 0, s, d, mode);
         else
-            hipLaunchKernelGGL((k_force<float, true>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+            hipLaunchKernelGGL((k_force<float, true>), grid, blk, 0, s, d, mode);
     } else {
         if (single_species)
-            hipLaunchKernelGGL((k_force<double, false>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+            hipLaunchKernelGGL((k_force<double, false>), grid, blk, 0, s, d, mode);
         else
-            hipLaunchKernelGGL((k_force<double, true>), dim3(nblk), dim3(BLK), 0, s, d, mode);
+            hipLaunchKernelGGL((k_force<double, true>), grid, blk, 0, s, d, mode);
     }
+    hipLaunchKernelGGL(k_traj_post, dim3(g.ntraj), dim3(BLK), 0, s, d, mode);
 }
 
 void launch_kick_nh(const DevPtrs &d, int nblk, bool mixed, hipStream_t s)
