@@ -75,6 +75,33 @@ def force_error(params, states, ntraj=4):
     return float(np.sqrt(num / den))
 
 
+def measure(name, precisions, steps, warmup=10, cluster_every=None, series_every=None):
+    """Throughput of one workload at each precision (same seeded start);
+    energy drift per tau of the extended Hamiltonian; largest-cluster series."""
+    params, states, meta = W.WORKLOADS[name]()
+    natoms = int(sum(p.natom for p in params))
+    every = meta.get("cluster_every", 0) if cluster_every is None else cluster_every
+    out = {"atoms": natoms, "trajectories": len(params), "steps": steps}
+    for prec in precisions:
+        tag = "mixed" if prec == fcmd.MIXED else "fp64"
+        with fcmd.System(params, prec) as s:
+            W.setup(s, params, states)
+            s.run(warmup, use_graph=True)
+            h0 = hamiltonian(s, params)
+            t0 = time.perf_counter()
+            series, _ = s.run(steps, cluster_every=series_every or every, use_graph=True)
+            wall = time.perf_counter() - t0
+            h1 = hamiltonian(s, params)
+            tau = steps * params[0].dt / W.TAU
+            out[tag] = {"atom_steps_per_s": natoms * steps / wall, "ms_per_step": wall / steps * 1e3,
+                        "energy_drift_per_tau": float(np.median(np.abs(h1 - h0) / np.abs(h0) / tau)),
+                        "largest_cluster_series": series.max(axis=1).tolist() if series.size else []}
+        log(f"[bench] {name} {tag}: {out[tag]['atom_steps_per_s']:.4e} atom-steps/s")
+    if fcmd.MIXED in precisions and fcmd.FP64 in precisions:
+        out["rms_rel_force_err"] = force_error(params, states, ntraj=1)
+    return out
+
+
 # --------------------------------------------------------------------------
 # CPU baseline: the reference's own plugin set (nose build: int-verlet-nh-mav,
 # neighbor-pbc-mav-silent, for-a-lj-n-pbc-m, stoddard-pbc-n-m) compiled from
@@ -132,6 +159,22 @@ def cpu_baseline(params, states, cores, budget_s=15.0):
                        "includes the initial O(N^2) list build")
 
 
+def copy_probe(mib):
+    """Five launches of an elementwise copy (dst = src * 1) of mib MiB: a
+    kernel whose HBM traffic is known exactly (mib MiB read, mib MiB
+    written), to scale FETCH_SIZE / WRITE_SIZE counter readings."""
+    import torch
+
+    n = (mib << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: 5 x {mib} MiB read + {mib} MiB written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,14 +186,22 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step's kernels on the stream instead of replaying a graph")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (262k FP64, 2M FP64 vs mixed, mixture)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy kernel (fp32, MIB MiB in and out) five "
+                         "times: the calibration for HBM byte counters")
     args = ap.parse_args()
     use_graph = not args.no_graph
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "trajectories shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+              "trajectories shard across GPUs with no communication", file=sys.stderr)
         sys.exit(2)
     if fcmd.device_count() < 1:
         raise SystemExit("bench.py: no HIP device")
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     K, Wm = args.steps, args.warmup
     log(f"[bench] building workload {args.workload}")
@@ -222,6 +273,14 @@ def main():
         traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
     step_bytes = (120 + 4 * nl) * natoms  # whole-step algorithmic bytes (mixed)
 
+    extra = None
+    if not args.no_extra:
+        extra = {
+            "lj_nh_262k": measure("lj_nh_262k", (fcmd.FP64, fcmd.MIXED), steps=1000),
+            "lj_nh_2m": measure("lj_nh_2m", (fcmd.FP64, fcmd.MIXED), steps=400, series_every=50),
+            "mixture64": measure("mixture64", (fcmd.MIXED,), steps=400),
+        }
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (reference Fortran on host cores)")
@@ -262,6 +321,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "bytes_per_atom": per_atom[dom], "list_entries_per_atom": nl},
         "cpu_baseline": cpu,
+        "other_workloads": extra,
     }
     print(json.dumps(out), flush=True)
 

@@ -229,14 +229,17 @@ int sync_err(fcmd_system *s)
     return FCMD_OK;
 }
 
-// one integrator step of every trajectory
+// one integrator step of every trajectory.  The Nose-Hoover final velocity
+// update is left pending and done by the next drift; finalize() applies it
+// wherever v is read or replaced outside the step sequence.
 void enqueue_step(fcmd_system *s)
 {
     launch_drift(s->d, s->g, s->mixed, s->stream);
     launch_rebuild(s->d, s->g, s->mixed, s->stream);
     launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
-    launch_kick_nh(s->d, s->g.nblk, s->mixed, s->stream);
 }
+
+void finalize(fcmd_system *s) { launch_kick_nh(s->d, s->g, s->mixed, s->stream); }
 
 }  // namespace
 
@@ -351,6 +354,8 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.blk_traj, s->g.nblk) A(d.cblk_traj, s->g.ncblk) A(d.tblk_traj, s->g.ntblk);
     A(d.tp, ntraj) A(d.ts, ntraj) A(d.cls, ntraj);
     A(d.part, (long long)s->g.nblk * NPART);
+    A(d.wl_blk, s->g.nblk) A(d.wl_cchk, s->g.ncblk) A(d.wl_tchk, s->g.ntblk) A(d.wl_traj, ntraj) A(d.wl_cnt, 4);
+    A(d.wl_off, 4LL * ntraj) A(d.nbtmp, N * NB_TMP);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, std::max(s->g.ncblk, s->g.ntblk));
     A(d.nn, N) A(d.tile_w, tbase) A(d.tile_off, tbase) A(d.list, lbase);
@@ -435,6 +440,7 @@ int fcmd_upload(fcmd_system *s, int traj, const double *x, const double *y, cons
                 const double *ay, const double *az, const int *api)
 {
     if (check_traj(s, traj)) return FCMD_ERR_ARG;
+    finalize(s);
     const TrajParams &t = s->tp[traj];
     const int n = t.natom;
     const dim3 grid((n + 255) / 256), blk(256);
@@ -462,6 +468,7 @@ int fcmd_download(fcmd_system *s, int traj, double *x, double *y, double *z, dou
                   double *vz, double *ax, double *ay, double *az)
 {
     if (check_traj(s, traj)) return FCMD_ERR_ARG;
+    finalize(s);
     const TrajParams &t = s->tp[traj];
     const int n = t.natom;
     const dim3 grid((n + 255) / 256), blk(256);
@@ -484,6 +491,8 @@ int fcmd_download(fcmd_system *s, int traj, double *x, double *y, double *z, dou
 int fcmd_set_scalars(fcmd_system *s, int traj, const fcmd_scalars *sc)
 {
     if (check_traj(s, traj) || !sc) return FCMD_ERR_ARG;
+    finalize(s);  // the pending update uses the current psi
+    HIPCHK(hipStreamSynchronize(s->stream));
     TrajScalars t;
     HIPCHK(hipMemcpy(&t, s->d.ts + traj, sizeof(t), hipMemcpyDeviceToHost));
     t.atrskin = sc->atrskin;
@@ -547,6 +556,7 @@ int fcmd_neighbor(fcmd_system *s, int traj, int *step)
 int fcmd_accatom(fcmd_system *s)
 {
     if (!s) return FCMD_ERR_ARG;
+    finalize(s);  // the pending update reads a, which AccAtom changes
     launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_ACCATOM, s->stream);
     HIPCHK(hipGetLastError());
     return sync_err(s);
@@ -560,6 +570,7 @@ int fcmd_integrator(fcmd_system *s, int *step)
         HIPCHK(hipMemcpyAsync((char *)(s->d.ts + k) + offsetof(TrajScalars, step), step, sizeof(int),
                               hipMemcpyHostToDevice, s->stream));
     enqueue_step(s);
+    finalize(s);  // integrator(step) returns with the step's final velocities
     HIPCHK(hipGetLastError());
     return sync_err(s);
 }
@@ -576,6 +587,7 @@ int fcmd_stoddard(fcmd_system *s, int traj, int step, int *cl, int *cs, int *cc,
     HIPCHK(hipMemcpy(cls.data(), s->d.cls, sizeof(ClusterScalars) * s->ntraj, hipMemcpyDeviceToHost));
     for (int k = 0; k < s->ntraj; ++k) cls[k].want = (k == traj);
     HIPCHK(hipMemcpy(s->d.cls, cls.data(), sizeof(ClusterScalars) * s->ntraj, hipMemcpyHostToDevice));
+    finalize(s);  // cluster kinetic energies need the final velocities
     launch_cluster(s->d, s->g, fre, s->stream);
     HIPCHK(hipGetLastError());
     int rc = sync_err(s);
@@ -722,6 +734,7 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
     int done = 0, snap = 0;
     while (done < nsteps) {
         if (nsnap > 0 && done % cluster_every == 0) {
+            finalize(s);
             launch_cluster(s->d, s->g, s->any_fre, s->stream);
             hipLaunchKernelGGL(k_record, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->d.ts,
                                s->ntraj, dseries + (size_t)snap * s->ntraj, dfpt, fpt_size, step0[0] + done);
@@ -735,6 +748,7 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
         HIPCHK(hipGetLastError());
         done += todo;
     }
+    finalize(s);
     HIPCHK(hipEventRecord(s->ev1, s->stream));
     HIPCHK(hipEventSynchronize(s->ev1));
     float ms = 0.f;
@@ -771,7 +785,9 @@ int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
         HIPCHK(hipEventRecord(e[2], s->stream));
         launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
         HIPCHK(hipEventRecord(e[3], s->stream));
-        launch_kick_nh(s->d, s->g.nblk, s->mixed, s->stream);
+        // the run loop defers the NH update to the next drift; only the last
+        // step of a run (or an analysis) pays it, so it is timed once
+        if (k == nsteps - 1) finalize(s);
         HIPCHK(hipEventRecord(e[4], s->stream));
     }
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -24,6 +24,7 @@ constexpr int TILE = 64;    // atoms per neighbour-list tile (one wavefront)
 constexpr int MAXS = 10;    // species, const.inc:12 mxsorts
 constexpr int EMAX = 48;    // cluster edges (r <= atskin) kept per atom
 constexpr int NPART = 4;    // per-block partial sums
+constexpr int NB_TMP = 48;  // neighbours per atom kept by the count pass
 
 // plugin variants (which reference file each entry point follows)
 enum { INT_VV = 0, INT_MAV = 1, INT_NH = 2 };     // int-verlet.f, -mav.f, -nh-mav.f
@@ -77,7 +78,8 @@ struct TrajScalars {
     double cside[3];                 // cell side used by the current list
     int dmmst, dmmi, step, rebuild, err, npairs, nrebuild, nlist;
     int ncx, ncy, ncz, pad0;
-    int nh_iters, pad1;
+    int nh_iters;
+    int pending;                     // v holds v_h: the NH kick (1+dth psi) is still to apply
 };
 
 // Device pointers of one engine.  Accelerations are double (FP64 path) or
@@ -99,6 +101,11 @@ struct DevPtrs {
     TrajParams *tp;
     TrajScalars *ts;
     double *part;                    // NPART per atom block
+    // rebuild work lists: atom blocks, cell chunks, tile chunks and
+    // trajectories of the trajectories that rebuild now; counts in wl_cnt[4]
+    int *wl_blk, *wl_cchk, *wl_tchk, *wl_traj, *wl_cnt;
+    int *wl_off;                     // 4 per trajectory: its offset in each list
+    int *nbtmp;                      // NB_TMP neighbours per atom slot (count pass)
     // cells
     int *cell_cnt, *cell_start, *atom_cell, *atom_slot, *sorted;
     long long *scan_tmp_ll;          // chunk totals for segmented scans

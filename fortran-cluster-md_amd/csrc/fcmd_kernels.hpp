@@ -20,7 +20,8 @@ struct LaunchDims {
 void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
                   hipStream_t s);
-void launch_kick_nh(const DevPtrs &d, int nblk, bool mixed, hipStream_t s);
+// applies any pending Nose-Hoover final update and clears the flags
+void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s);
 

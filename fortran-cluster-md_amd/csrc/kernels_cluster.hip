@@ -238,33 +238,58 @@ __global__ __launch_bounds__(BLK) void k_cl_rank(DevPtrs d)
 
 __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
 {
+    __shared__ double ldsd[4];
+    __shared__ int ldsi[4];
     const int b = blockIdx.x;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     if (!d.cls[t].want || d.ts[t].err) return;
     const int i = b * BLK + threadIdx.x;
-    if (i - tp.abase >= tp.natom) return;
+    const bool act = i - tp.abase < tp.natom;
     ClusterScalars &c = d.cls[t];
+    // monomers dominate a vapour: their histogram row is summed per block
+    // (one atomic per block, not per atom)
+    int mono = 0, pos = 0, big = 0;
+    double mono_ke = 0.0;
+    if (act) {
+        const int oi = d.oid[i];
+        const int o = tp.abase + oi - 1;
+        const bool root = ld_relaxed(d.parent + i) == i;
+        if (root) {
+            const int sz = d.cs_tmp[i];
+            d.o_cs[o] = sz;
+            d.o_cek[o] = d.cek_tmp[i];
+            d.o_cgs[o] = d.cgs_tmp[i];
+            if (sz == 1) {
+                mono = 1;
+                mono_ke = d.cek_tmp[i];
+            } else if (sz > 1) {
+                atomicAdd(d.o_csi + tp.hbase + sz, 1);
+                atomicAdd(d.o_ceks + tp.hbase + sz, d.cek_tmp[i]);
+            }
+            pos = sz > 0;
+            big = sz;
+        } else {
+            d.o_cs[o] = 0;
+            d.o_cek[o] = 0.0;
+            d.o_cgs[o] = 0;
+        }
+    }
+    const int bm = block_sum_i(mono, ldsi);
+    const int bp = block_sum_i(pos, ldsi);
+    const double be = block_sum(mono_ke, ldsd);
+    const int bb = wave_max_i(big);
+    if ((threadIdx.x & 63) == 0 && bb > 0) atomicMax(&c.cbig, bb);
+    if (threadIdx.x == 0) {
+        if (bm) {
+            atomicAdd(d.o_csi + tp.hbase + 1, bm);
+            atomicAdd(d.o_ceks + tp.hbase + 1, be);
+        }
+        if (bp) atomicAdd(&c.npos, bp);
+    }
+    if (!act || tp.stoddard != ST_FRE) return;
     const int oi = d.oid[i];
     const int o = tp.abase + oi - 1;
-    const bool root = ld_relaxed(d.parent + i) == i;
-    if (root) {
-        const int sz = d.cs_tmp[i];
-        d.o_cs[o] = sz;
-        d.o_cek[o] = d.cek_tmp[i];
-        d.o_cgs[o] = d.cgs_tmp[i];
-        if (sz > 0) {
-            atomicAdd(d.o_csi + tp.hbase + sz, 1);
-            atomicAdd(d.o_ceks + tp.hbase + sz, d.cek_tmp[i]);
-            atomicAdd(&c.npos, 1);
-            atomicMax(&c.cbig, sz);
-        }
-    } else {
-        d.o_cs[o] = 0;
-        d.o_cek[o] = 0.0;
-        d.o_cgs[o] = 0;
-    }
-    if (tp.stoddard != ST_FRE) return;
     if (d.liq[i]) {
         const int rf = uf_find(d.parentf, i);
         const int label = d.rank[tp.abase + d.oid[rf] - 1] + 1;

@@ -101,6 +101,7 @@ __device__ void reduce_partials(const TrajParams &tp, const double *part, int s0
 __device__ void traj_pre(const TrajParams &tp, TrajScalars &ts, double dmmr, double sumv2)
 {
     ts.rebuild = 0;
+    ts.pending = 0;  // the drift has applied it
     if (tp.integrator == INT_NH) {
         ts.sumv2 = sumv2;
         if (tp.ethst_nh) {
@@ -182,6 +183,14 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
             const int ty = d.typ[i];
             const bool cpl = nh_coupled(tp, ty);
             const double psi = cpl ? ts.psi : 0.0;
+            if (ts.pending) {
+                // the previous step's final update, deferred to here (same
+                // expression as k_kick_nh, so the same bits)
+                const double den = cpl ? 1.0 + dth * ts.psi : 1.0;
+                vx = (vx + dth * ax) / den;
+                vy = (vy + dth * ay) / den;
+                vz = (vz + dth * az) / den;
+            }
             dx = dt * vx + dt2 * (ax - psi * vx);
             dy = dt * vy + dt2 * (ay - psi * vy);
             dz = dt * vz + dt2 * (az - psi * vz);
@@ -292,6 +301,7 @@ __device__ void traj_post(const TrajParams &tp, TrajScalars &ts, double esum, do
     if (tp.integrator == INT_NH) {
         nh_solve(tp, ts, eks);
         ts.ekin += ekfree;
+        ts.pending = 1;  // v' = (v_h + dth a)/(1 + dth psi') is applied later
     } else {
         ts.ekin = eks;
     }
@@ -338,49 +348,67 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
     KahanF kx, ky, kz;
     const float rc2f = (float)rc2;
     fx = fy = fz = 0.0;
-    for (int s = 0; s < nn; ++s) {
-        const int j = lp[(size_t)s * TILE];
-        double dx = xi - d.x[j];
-        double dy = yi - d.y[j];
-        double dz = zi - d.z[j];
-        dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
-        dy = dy + by * (bf1 - fint(bf2 + dy * iby));
-        dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
-        const int pij = (FORM == ACC_1) ? 0 : ti * MAXS + d.typ[j];
-        if (!MIXED) {
-            double rij2 = dx * dx + dy * dy + dz * dz;
-            if (rij2 <= rc2) {
-                rij2 = 1.0 / rij2;
-                const double rij6 = rij2 * rij2 * rij2;
-                double fij;
-                if (FORM == ACC_1) {
-                    fij = rij2 * rij6 * (tp.c2[0] * rij6 - tp.c3[0]);
-                    e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
-                } else {
-                    fij = tp.e48[pij] * rij2 * rij6 * (tp.c2[pij] * rij6 - tp.c3[pij]);
-                    e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
+    // neighbours in chunks of four: all index and position loads of a chunk
+    // are issued before its arithmetic; pairs are still summed in list order
+    constexpr int U = 4;
+    for (int s0 = 0; s0 < nn; s0 += U) {
+        int jj[U];
+        double xj[U], yj[U], zj[U];
+        int tj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? lp[(size_t)(s0 + u) * TILE] : i;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xj[u] = d.x[jj[u]];
+            yj[u] = d.y[jj[u]];
+            zj[u] = d.z[jj[u]];
+            tj[u] = (FORM == ACC_1) ? 0 : d.typ[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (s0 + u >= nn) break;
+            double dx = xi - xj[u];
+            double dy = yi - yj[u];
+            double dz = zi - zj[u];
+            dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
+            dy = dy + by * (bf1 - fint(bf2 + dy * iby));
+            dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
+            const int pij = (FORM == ACC_1) ? 0 : ti * MAXS + tj[u];
+            if (!MIXED) {
+                double rij2 = dx * dx + dy * dy + dz * dz;
+                if (rij2 <= rc2) {
+                    rij2 = 1.0 / rij2;
+                    const double rij6 = rij2 * rij2 * rij2;
+                    double fij;
+                    if (FORM == ACC_1) {
+                        fij = rij2 * rij6 * (tp.c2[0] * rij6 - tp.c3[0]);
+                        e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
+                    } else {
+                        fij = tp.e48[pij] * rij2 * rij6 * (tp.c2[pij] * rij6 - tp.c3[pij]);
+                        e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
+                    }
+                    fx = fx + fij * dx;
+                    fy = fy + fij * dy;
+                    fz = fz + fij * dz;
                 }
-                fx = fx + fij * dx;
-                fy = fy + fij * dy;
-                fz = fz + fij * dz;
-            }
-        } else {
-            const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
-            const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
-            if (r2 <= rc2f) {
-                const float ri2 = 1.0f / r2;
-                const float r6 = ri2 * ri2 * ri2;
-                float fij;
-                if (FORM == ACC_1) {
-                    fij = ri2 * r6 * (tp.c2f[0] * r6 - tp.c3f[0]);
-                    e += (double)(r6 * (tp.c2f[0] * r6 - tp.c1f[0]));
-                } else {
-                    fij = (float)tp.e48[pij] * ri2 * r6 * (tp.c2f[pij] * r6 - tp.c3f[pij]);
-                    e += tp.e4[pij] * (double)(r6 * (tp.c2f[pij] * r6 - tp.c1f[pij]));
+            } else {
+                const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+                const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
+                if (r2 <= rc2f) {
+                    const float ri2 = 1.0f / r2;
+                    const float r6 = ri2 * ri2 * ri2;
+                    float fij;
+                    if (FORM == ACC_1) {
+                        fij = ri2 * r6 * (tp.c2f[0] * r6 - tp.c3f[0]);
+                        e += (double)(r6 * (tp.c2f[0] * r6 - tp.c1f[0]));
+                    } else {
+                        fij = (float)tp.e48[pij] * ri2 * r6 * (tp.c2f[pij] * r6 - tp.c3f[pij]);
+                        e += tp.e4[pij] * (double)(r6 * (tp.c2f[pij] * r6 - tp.c1f[pij]));
+                    }
+                    kx.add(fij * dxf);
+                    ky.add(fij * dyf);
+                    kz.add(fij * dzf);
                 }
-                kx.add(fij * dxf);
-                ky.add(fij * dyf);
-                kz.add(fij * dzf);
             }
         }
     }
@@ -468,7 +496,9 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
     }
 }
 
-// Nose-Hoover final velocities v' = (v_h + dth a)/(1 + dth psi').
+// Nose-Hoover final velocities v' = (v_h + dth a)/(1 + dth psi'), for
+// trajectories whose update is still pending (the run loop leaves it to the
+// next drift; everything that reads v calls this first).
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_kick_nh(DevPtrs d)
 {
@@ -476,7 +506,7 @@ __global__ __launch_bounds__(BLK) void k_kick_nh(DevPtrs d)
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     const TrajScalars &ts = d.ts[t];
-    if (ts.err || tp.integrator != INT_NH) return;
+    if (ts.err || tp.integrator != INT_NH || !ts.pending) return;
     const int i = b * BLK + threadIdx.x;
     if (i - tp.abase >= tp.natom) return;
     const double dth = tp.dt / 2;
@@ -534,12 +564,19 @@ void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single
     hipLaunchKernelGGL(k_traj_post, dim3(g.ntraj), dim3(BLK), 0, s, d, mode);
 }
 
-void launch_kick_nh(const DevPtrs &d, int nblk, bool mixed, hipStream_t s)
+__global__ void k_clear_pending(DevPtrs d, int ntraj)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntraj) d.ts[t].pending = 0;
+}
+
+void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
     if (mixed)
-        hipLaunchKernelGGL(k_kick_nh<float>, dim3(nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_kick_nh<float>, dim3(g.nblk), dim3(BLK), 0, s, d);
     else
-        hipLaunchKernelGGL(k_kick_nh<double>, dim3(nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_kick_nh<double>, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_clear_pending, dim3((g.ntraj + 255) / 256), dim3(256), 0, s, d, g.ntraj);
 }
 
 }  // namespace fcmd

@@ -137,13 +137,26 @@ def nh_initial_psi(p: Params, st: dict) -> float:
     return (2.0 * ek - 3.0 * sel.sum() * CIKB * p.att) / p.Q
 
 
+def kinetic_energy(p: Params, st: dict) -> float:
+    """ekin as the integrators define it: 0.5 m1 sum v^2 for the reference
+    Nose-Hoover (int-verlet-nh-mav.f:180), per-species masses otherwise
+    (int-verlet-mav.f:154)."""
+    v2 = st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2
+    if p.integrator == INT_NH and not p.nh_mask:
+        return 0.5 * p.mas[0] * v2.sum()
+    m = np.array([p.mas[s] for s in range(p.nsorts)])[st["api"] - 1]
+    return 0.5 * (m * v2).sum()
+
+
 def setup(system, params, states):
     """Upload states, build every list (neighbor(0)), compute a(0)
-    (AccAtom) and set the initial thermostat friction."""
+    (AccAtom), and set ekin and the initial thermostat friction."""
     for t, (p, st) in enumerate(zip(params, states)):
         system.upload(st, traj=t)
         system.neighbor(0, traj=t)
     system.accatom()
     for t, (p, st) in enumerate(zip(params, states)):
+        kw = dict(ekin=kinetic_energy(p, st))
         if p.integrator == INT_NH and p.ethst_nh:
-            system.set_scalars(t, psi=nh_initial_psi(p, st), s=0.0)
+            kw.update(psi=nh_initial_psi(p, st), s=0.0)
+        system.set_scalars(t, **kw)
