@@ -159,22 +159,6 @@ def cpu_baseline(params, states, cores, budget_s=15.0):
                        "includes the initial O(N^2) list build")
 
 
-def copy_probe(mib):
-    """Five launches of an elementwise copy (dst = src * 1) of mib MiB: a
-    kernel whose HBM traffic is known exactly (mib MiB read, mib MiB
-    written), to scale FETCH_SIZE / WRITE_SIZE counter readings."""
-    import torch
-
-    n = (mib << 20) // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    for _ in range(5):
-        torch.mul(src, 1.0, out=dst)
-    torch.cuda.synchronize()
-    log(f"[bench] copy probe: 5 x {mib} MiB read + {mib} MiB written")
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,9 +172,6 @@ def main():
                     help="launch every step's kernels on the stream instead of replaying a graph")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (262k FP64, 2M FP64 vs mixed, mixture)")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy kernel (fp32, MIB MiB in and out) five "
-                         "times: the calibration for HBM byte counters")
     args = ap.parse_args()
     use_graph = not args.no_graph
     if args.gpus != 1:
@@ -199,9 +180,6 @@ def main():
         sys.exit(2)
     if fcmd.device_count() < 1:
         raise SystemExit("bench.py: no HIP device")
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
 
     K, Wm = args.steps, args.warmup
     log(f"[bench] building workload {args.workload}")
@@ -217,16 +195,20 @@ def main():
         s = build_system(params, states, prec)
         s.run(Wm, cluster_every=0, use_graph=use_graph)
         h0 = hamiltonian(s, params)
+        nr0 = np.array([s.scalars(t).nrebuild for t in range(len(params))])
         t0 = time.perf_counter()
         series, fpt = s.run(K, cluster_every=every, fpt_size=fpt_size, use_graph=use_graph)
         wall = time.perf_counter() - t0
         h1 = hamiltonian(s, params)
+        nr1 = np.array([s.scalars(t).nrebuild for t in range(len(params))])
+        sizes = np.array([p.natom for p in params])
         ev_ms = s.last_run_ms()
         tau = K * params[0].dt / W.TAU
         hdrift[name] = np.abs(h1 - h0) / np.abs(h0) / tau
         res[name] = dict(wall=wall, ev_ms=ev_ms, rate=natoms * K / wall,
                          nlist=s.list_entries(), largest=int(series.max()) if series.size else 0,
-                         nfpt=int((fpt >= 0).sum()))
+                         nfpt=int((fpt >= 0).sum()),
+                         rebuild_frac=float((sizes * (nr1 - nr0)).sum() / (natoms * K)))
         log(f"[bench] {name}: {K} steps in {wall:.3f} s (events {ev_ms / 1e3:.3f} s) -> "
             f"{res[name]['rate']:.4e} atom-steps/s")
         if name == "mixed":
@@ -314,6 +296,7 @@ def main():
         "rms_rel_force_err": ferr,
         "energy_drift_ratio_mixed_over_fp64": drift_ratio,
         "largest_cluster": res["mixed"]["largest"],
+        "rebuilt_atom_fraction_per_step": res["mixed"]["rebuild_frac"],
         "trajectories_reaching_fpt": res["mixed"]["nfpt"],
         "kernel_ms_per_step": kt,
         "step_roofline_frac": step_bytes * K / res["mixed"]["wall"] / HBM_PEAK,

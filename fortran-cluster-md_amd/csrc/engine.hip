@@ -98,7 +98,7 @@ struct fcmd_system {
     std::vector<fcmd_params> hp;
     std::vector<TrajParams> tp;
     LaunchDims g{};
-    long long natoms_total = 0, ncells_total = 0, ntiles_total = 0, nlist_total = 0, nhist_total = 0;
+    long long natoms_total = 0, ncells_total = 0, nlist_total = 0, nhist_total = 0;
     bool single_species = true;
     bool any_fre = false;
     DevPtrs d{};
@@ -282,7 +282,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     s->hp.assign(params, params + ntraj);
     s->tp.resize(ntraj);
     int rc;
-    long long abase = 0, cbase = 0, tbase = 0, lbase = 0, hbase = 0;
+    long long abase = 0, cbase = 0, lbase = 0, hbase = 0;
     for (int k = 0; k < ntraj; ++k) {
         const fcmd_params &p = params[k];
         TrajParams &t = s->tp[k];
@@ -299,8 +299,6 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         long long nc = 1;
         for (int d = 0; d < 3; ++d) nc *= std::max<long long>(1, (long long)std::floor(p.box[d] / smin));
         const long long ncpad = roundup(nc, SCAN_CHUNK);
-        const long long ntile = npad / TILE;
-        const long long ntpad = roundup(ntile, SCAN_CHUNK);
         t.abase = (int)abase;
         t.npad = (int)npad;
         t.bfirst = (int)(abase / BLK);
@@ -309,17 +307,12 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         t.ncpad = (int)ncpad;
         t.cfirst = (int)(cbase / SCAN_CHUNK);
         t.ncblk = (int)(ncpad / SCAN_CHUNK);
-        t.tbase = (int)tbase;
-        t.ntile = (int)ntile;
-        t.tfirst = (int)(tbase / SCAN_CHUNK);
-        t.ntblk = (int)(ntpad / SCAN_CHUNK);
         t.lbase = lbase;
-        t.lcap = npad * s->list_width;
+        t.lwidth = s->list_width;
         t.hbase = (int)hbase;
         abase += npad;
         cbase += ncpad;
-        tbase += ntpad;
-        lbase += t.lcap;
+        lbase += npad * s->list_width;
         hbase += p.natom + 1;
     }
     if (abase >= (1LL << 31) || cbase >= (1LL << 31) || lbase >= (1LL << 40)) {
@@ -328,16 +321,15 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     }
     s->natoms_total = abase;
     s->ncells_total = cbase;
-    s->ntiles_total = tbase;
     s->nlist_total = lbase;
     s->nhist_total = hbase;
     s->g.nblk = (int)(abase / BLK);
     s->g.ncblk = (int)(cbase / SCAN_CHUNK);
-    s->g.ntblk = (int)(tbase / SCAN_CHUNK);
     s->g.ntraj = ntraj;
 
     DevPtrs &d = s->d;
     const long long N = abase;
+    d.nblk = s->g.nblk;
 #define A(ptr, n)                                  \
     if ((rc = s->alloc(&(ptr), (n))) != FCMD_OK) { \
         delete s;                                  \
@@ -351,14 +343,14 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         A(d.ax, N) A(d.ay, N) A(d.az, N) A(d.ax2, N) A(d.ay2, N) A(d.az2, N);
     }
     A(d.typ, N) A(d.oid, N) A(d.typ2, N) A(d.oid2, N) A(d.inv, N);
-    A(d.blk_traj, s->g.nblk) A(d.cblk_traj, s->g.ncblk) A(d.tblk_traj, s->g.ntblk);
+    A(d.blk_traj, s->g.nblk) A(d.cblk_traj, s->g.ncblk);
     A(d.tp, ntraj) A(d.ts, ntraj) A(d.cls, ntraj);
     A(d.part, (long long)s->g.nblk * NPART);
-    A(d.wl_blk, s->g.nblk) A(d.wl_cchk, s->g.ncblk) A(d.wl_tchk, s->g.ntblk) A(d.wl_traj, ntraj) A(d.wl_cnt, 4);
-    A(d.wl_off, 4LL * ntraj) A(d.nbtmp, N * NB_TMP);
+    A(d.wl_blk, s->g.nblk) A(d.wl_cchk, s->g.ncblk) A(d.wl_traj, ntraj) A(d.wl_cnt, WL_LISTS);
+    A(d.wl_size, (long long)WL_LISTS * ntraj);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
-    A(d.scan_tmp_ll, std::max(s->g.ncblk, s->g.ntblk));
-    A(d.nn, N) A(d.tile_w, tbase) A(d.tile_off, tbase) A(d.list, lbase);
+    A(d.scan_tmp_ll, s->g.ncblk);
+    A(d.nn, N) A(d.list, lbase);
     A(d.edeg, N) A(d.edeg1, N) A(d.edges, N * EMAX) A(d.parent, N) A(d.parentf, N) A(d.liq, N);
     A(d.cs_tmp, N) A(d.cgs_tmp, N) A(d.csf_tmp, N) A(d.cek_tmp, N) A(d.cekf_tmp, N);
     A(d.liqo, N) A(d.rank, N) A(d.o_cl, N) A(d.o_cs, N) A(d.o_cgs, N) A(d.o_clf, N) A(d.o_ccf, N);
@@ -370,12 +362,14 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(s->stage, maxn) A(s->stage_i, maxn) A(s->dstep, 4);
 #undef A
 
-    std::vector<int> bt(s->g.nblk), ct(s->g.ncblk), tt(s->g.ntblk);
+    std::vector<int> bt(s->g.nblk), ct(s->g.ncblk), wsz((size_t)WL_LISTS * ntraj);
     for (int k = 0; k < ntraj; ++k) {
         const TrajParams &t = s->tp[k];
         for (int b = 0; b < t.nblk; ++b) bt[t.bfirst + b] = k;
         for (int b = 0; b < t.ncblk; ++b) ct[t.cfirst + b] = k;
-        for (int b = 0; b < t.ntblk; ++b) tt[t.tfirst + b] = k;
+        wsz[WL_LISTS * k + WL_BLK] = t.nblk;
+        wsz[WL_LISTS * k + WL_CCHK] = t.ncblk;
+        wsz[WL_LISTS * k + WL_TRAJ] = 1;
     }
     if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
         hipEventCreate(&s->ev1) != hipSuccess) {
@@ -384,7 +378,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     }
     if (hipMemcpy(d.blk_traj, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.cblk_traj, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d.tblk_traj, tt.data(), sizeof(int) * tt.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.wl_size, wsz.data(), sizeof(int) * wsz.size(), hipMemcpyHostToDevice) != hipSuccess ||
         upload_params(s) != FCMD_OK) {
         delete s;
         return FCMD_ERR_HIP;
@@ -655,23 +649,17 @@ int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *at
     const TrajParams &t = s->tp[traj];
     HIPCHK(hipStreamSynchronize(s->stream));
     const int n = t.natom;
-    std::vector<int> nn(t.npad), oid(t.npad), tw(t.ntile);
-    std::vector<long long> toff(t.ntile);
-    TrajScalars ts;
-    HIPCHK(hipMemcpy(&ts, s->d.ts + traj, sizeof(ts), hipMemcpyDeviceToHost));
+    std::vector<int> nn(t.npad), oid(t.npad);
     HIPCHK(hipMemcpy(nn.data(), s->d.nn + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(oid.data(), s->d.oid + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(toff.data(), s->d.tile_off + t.tbase, sizeof(long long) * t.ntile, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(tw.data(), s->d.tile_w + t.tbase, sizeof(int) * t.ntile, hipMemcpyDeviceToHost));
-    long long lo = toff.empty() ? 0 : toff[0], hi = lo;
-    for (int k = 0; k < t.ntile; ++k) hi = std::max(hi, toff[k] + (long long)tw[k] * TILE);
-    std::vector<int> lst(std::max<long long>(hi - lo, 1));
-    if (hi > lo) HIPCHK(hipMemcpy(lst.data(), s->d.list + lo, sizeof(int) * (hi - lo), hipMemcpyDeviceToHost));
+    const long long lo = t.lbase, nslots = (long long)t.npad * t.lwidth;
+    std::vector<int> lst(nslots);
+    HIPCHK(hipMemcpy(lst.data(), s->d.list + lo, sizeof(int) * nslots, hipMemcpyDeviceToHost));
     // neighbours (reference numbers, j > i) of each reference atom i
     std::vector<std::vector<int>> nb(n);
     for (int r = 0; r < n; ++r) {
         const int oi = oid[r];
-        const long long base = toff[r / TILE] - lo + (r % TILE);
+        const long long base = list_slot(t, r) - lo;
         for (int q = 0; q < nn[r]; ++q) {
             const int j = lst[base + (long long)q * TILE] - t.abase;
             const int oj = oid[j];
@@ -773,22 +761,28 @@ int fcmd_last_run_ms(fcmd_system *s, double *total_ms)
 int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
 {
     if (!s || nsteps <= 0 || !avg_ms) return FCMD_ERR_ARG;
-    const int nk = 4;  // drift, rebuild sequence, force, kick
+    const int nk = FCMD_NTIMED;  // k_drift, k_traj_pre, rebuild sequence, k_force, k_traj_post, kick
     std::vector<hipEvent_t> ev((size_t)nsteps * (nk + 1));
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    // timing-only events: no system-scope fence (cache writeback) at each
+    // record, which would otherwise be charged to the bracketed kernel
+    for (auto &e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     for (int k = 0; k < nsteps; ++k) {
         hipEvent_t *e = ev.data() + (size_t)k * (nk + 1);
         HIPCHK(hipEventRecord(e[0], s->stream));
-        launch_drift(s->d, s->g, s->mixed, s->stream);
+        launch_drift_atoms(s->d, s->g, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[1], s->stream));
-        launch_rebuild(s->d, s->g, s->mixed, s->stream);
+        launch_traj_pre(s->d, s->g, s->stream);
         HIPCHK(hipEventRecord(e[2], s->stream));
-        launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        launch_rebuild(s->d, s->g, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[3], s->stream));
+        launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        HIPCHK(hipEventRecord(e[4], s->stream));
+        launch_traj_post(s->d, s->g, FORCE_STEP, s->stream);
+        HIPCHK(hipEventRecord(e[5], s->stream));
         // the run loop defers the NH update to the next drift; only the last
         // step of a run (or an analysis) pays it, so it is timed once
         if (k == nsteps - 1) finalize(s);
-        HIPCHK(hipEventRecord(e[4], s->stream));
+        HIPCHK(hipEventRecord(e[6], s->stream));
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int c = 0; c < nk; ++c) avg_ms[c] = 0.0;

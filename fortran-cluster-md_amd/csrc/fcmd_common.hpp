@@ -5,8 +5,11 @@
 
 namespace fcmd {
 
-// Fortran INT() of a double: truncation toward zero.
-__device__ __forceinline__ double fint(double v) { return (double)(int)v; }
+// Fortran INT() of a double, back in double: truncation toward zero.  One
+// v_trunc_f64 instead of a convert to int and back; the two agree for
+// |v| < 2^31, which every argument here (ratios of lengths to the box side or
+// to sigma) satisfies by far.
+__device__ __forceinline__ double fint(double v) { return trunc(v); }
 
 __device__ __forceinline__ double wave_sum(double v)
 {

@@ -2,10 +2,10 @@
 //
 // One allocation holds an ensemble of independent trajectories (a single
 // system is an ensemble of one).  Trajectory k owns
-//   atoms  [abase, abase + npad)   npad = natom rounded up to BLK
-//   cells  [cbase, cbase + ncpad)  enough for the smallest possible cell side
-//   tiles  [abase/TILE, ...)       one 64-atom tile per wavefront
-//   list   [lbase, lbase + lcap)   sliced-ELL neighbour list (width per tile)
+//   atoms  [abase, abase + npad)           npad = natom rounded up to BLK
+//   cells  [cbase, cbase + ncpad)          enough for the smallest cell side
+//   list   [lbase, lbase + npad * lwidth)  sliced-ELL neighbour list: per
+//          64-atom tile (one wavefront) lwidth slots x 64 lanes, slot-major
 // so that every thread block belongs to exactly one trajectory and every
 // per-trajectory reduction is a fixed-order sum over its own blocks.
 //
@@ -24,7 +24,20 @@ constexpr int TILE = 64;    // atoms per neighbour-list tile (one wavefront)
 constexpr int MAXS = 10;    // species, const.inc:12 mxsorts
 constexpr int EMAX = 48;    // cluster edges (r <= atskin) kept per atom
 constexpr int NPART = 4;    // per-block partial sums
-constexpr int NB_TMP = 48;  // neighbours per atom kept by the count pass
+constexpr int NXCD = 8;     // XCDs of one MI355X, each with its own L2
+
+// XCD-aware block order.  Workgroup IDs are handed to the XCDs round-robin;
+// logical block (p % NXCD) * chunk + p / NXCD for workgroup p gives every XCD
+// one contiguous run of blocks.  Atoms are sorted by cell, so the neighbour
+// gathers of a block then hit the L2 that has just loaded its neighbours in
+// space, instead of missing into memory from another XCD.  Launch
+// xcd_grid(n) workgroups for n logical blocks; ids >= n have no block.
+__host__ __device__ inline int xcd_block(int p, int n)
+{
+    const int chunk = (n + NXCD - 1) / NXCD;
+    return (p % NXCD) * chunk + p / NXCD;
+}
+__host__ __device__ inline int xcd_grid(int n) { return (n + NXCD - 1) / NXCD * NXCD; }
 
 // plugin variants (which reference file each entry point follows)
 enum { INT_VV = 0, INT_MAV = 1, INT_NH = 2 };     // int-verlet.f, -mav.f, -nh-mav.f
@@ -51,12 +64,22 @@ struct TrajParams {
     int integrator, neighbor, accatom, stoddard;
     int abase, npad, bfirst, nblk;   // atoms, atom blocks
     int cbase, ncpad, cfirst, ncblk; // cells, cell blocks
-    int tbase, ntile, tfirst, ntblk; // tiles, tile blocks (BLK tiles per block)
-    long long lbase, lcap;           // neighbour-list segment
+    long long lbase;                 // neighbour-list segment: npad * lwidth slots
+    int lwidth;                      // list slots per atom (its maximum neighbour count)
     int hbase;                       // cluster-size histograms, natom+1 entries
     int nh_mask;                     // species bits coupled to Nose-Hoover
     int nh_refmass;                  // 1: m1 for every coupled atom (reference)
 };
+
+// work lists of the rebuild
+enum { WL_BLK = 0, WL_CCHK = 1, WL_TRAJ = 2, WL_LISTS = 3 };
+
+// first list slot of trajectory-relative atom slot rel; its k-th neighbour is
+// at + k*TILE (the 64 lanes of a tile read 256 contiguous bytes per slot)
+__host__ __device__ inline long long list_slot(const TrajParams &tp, int rel)
+{
+    return tp.lbase + (long long)(rel / TILE) * tp.lwidth * TILE + (rel & (TILE - 1));
+}
 
 // per-trajectory results of one cluster analysis
 struct ClusterScalars {
@@ -95,23 +118,21 @@ struct DevPtrs {
     int *typ2, *oid2;
     int *inv;                        // inv[abase + oid - 1] = slot
     // per block / per trajectory
+    int nblk;                        // atom blocks of the whole arena
     int *blk_traj;                   // atom block -> trajectory
     int *cblk_traj;                  // cell block -> trajectory
-    int *tblk_traj;                  // tile block -> trajectory
     TrajParams *tp;
     TrajScalars *ts;
     double *part;                    // NPART per atom block
-    // rebuild work lists: atom blocks, cell chunks, tile chunks and
-    // trajectories of the trajectories that rebuild now; counts in wl_cnt[4]
-    int *wl_blk, *wl_cchk, *wl_tchk, *wl_traj, *wl_cnt;
-    int *wl_off;                     // 4 per trajectory: its offset in each list
-    int *nbtmp;                      // NB_TMP neighbours per atom slot (count pass)
+    // rebuild work lists: the atom blocks, cell chunks and trajectories of the
+    // trajectories that rebuild now; lengths in wl_cnt[WL_LISTS]
+    int *wl_blk, *wl_cchk, *wl_traj, *wl_cnt;
+    int *wl_size;                    // WL_LISTS per trajectory: its item counts
     // cells
     int *cell_cnt, *cell_start, *atom_cell, *atom_slot, *sorted;
     long long *scan_tmp_ll;          // chunk totals for segmented scans
-    // neighbour list
-    int *nn, *tile_w;
-    long long *tile_off;
+    // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE]
+    int *nn;
     int *list;
     // clusters: per slot
     ClusterScalars *cls;

@@ -5,21 +5,27 @@
 
 namespace fcmd {
 
-constexpr int SCAN_CHUNK = 4 * BLK;  // cells / tiles per scan chunk
+constexpr int SCAN_CHUNK = 4 * BLK;  // cells per scan chunk
 
 enum { FORCE_STEP = 0, FORCE_ACCATOM = 1 };
 
 struct LaunchDims {
     int nblk;    // atom blocks
     int ncblk;   // cell chunks
-    int ntblk;   // tile chunks
     int nhblk;   // histogram chunks
     int ntraj;
 };
 
+// launch_drift = launch_drift_atoms (k_drift) + launch_traj_pre (k_traj_pre);
+// launch_force = launch_force_atoms (k_force) + launch_traj_post (k_traj_post)
 void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
+void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
+void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s);
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
                   hipStream_t s);
+void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
+                        hipStream_t s);
+void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s);
 // applies any pending Nose-Hoover final update and clears the flags
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);

@@ -59,7 +59,8 @@ __device__ __forceinline__ double kin(const DevPtrs &d, const TrajParams &tp, in
 // reset per-slot work arrays and this trajectory's histograms
 __global__ __launch_bounds__(BLK) void k_cl_init(DevPtrs d)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     if (!d.cls[t].want || d.ts[t].err) return;
@@ -100,7 +101,8 @@ __global__ __launch_bounds__(BLK) void k_cl_init(DevPtrs d)
 // edges within atskin; degrees; liquid-like flag (tWF)
 __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     const TrajScalars &ts = d.ts[t];
@@ -115,8 +117,7 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
     const double bf1 = fint(ts.atrskin / tp.atr) + 2.0;
     const double bf2 = bf1 + 0.5;
     const double rijc2 = tp.atskin * tp.atskin;
-    const int lane = threadIdx.x & (TILE - 1);
-    const int *lp = d.list + d.tile_off[tp.tbase + rel / TILE] + lane;
+    const int *lp = d.list + list_slot(tp, rel);
     const int nn = d.nn[i];
     int deg = 0, deg1 = 0;
     int *ed = d.edges + (size_t)i * EMAX;
@@ -159,7 +160,8 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
 
 __global__ __launch_bounds__(BLK) void k_cl_hook(DevPtrs d)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     if (!d.cls[t].want || d.ts[t].err) return;
@@ -181,7 +183,8 @@ __global__ __launch_bounds__(BLK) void k_cl_hook(DevPtrs d)
 
 __global__ __launch_bounds__(BLK) void k_cl_count(DevPtrs d)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     if (!d.cls[t].want || d.ts[t].err) return;
@@ -240,7 +243,8 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
 {
     __shared__ double ldsd[4];
     __shared__ int ldsi[4];
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     if (!d.cls[t].want || d.ts[t].err) return;
@@ -323,12 +327,12 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
 
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cl_init, dim3(g.nblk), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_edges, dim3(g.nblk), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_hook, dim3(g.nblk), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_count, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_init, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_edges, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_hook, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_count, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     if (fre) hipLaunchKernelGGL(k_cl_rank, dim3(g.ntraj), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_finish, dim3(g.nblk), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_finish, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
 }
 
 }  // namespace fcmd

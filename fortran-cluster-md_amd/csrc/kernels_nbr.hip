@@ -2,16 +2,15 @@
 //
 // Replaces the O(N^2) half-list build of neighbor-pbc.f:62-94 /
 // neighbor-pbc-mav.f:197-229.  The drift kernel's per-trajectory epilogue
-// raises a trajectory's rebuild flag; k_worklist then lists the atom blocks,
-// cell chunks and tile chunks of the trajectories that rebuild now, and every
-// rebuild kernel loops over that list with a fixed grid.  A step in which no
-// trajectory rebuilds costs a few empty launches, and the sequence can be
-// captured in a graph because no host decision is involved.
+// raises a trajectory's rebuild flag; k_wl_fill then lists the atom blocks,
+// cell chunks and trajectories of the trajectories that rebuild now, and
+// every rebuild kernel loops over those lists with a fixed grid.  A step in
+// which no trajectory rebuilds costs a few empty launches, and the sequence
+// can be captured in a graph because no host decision is involved.
 //
-//   worklist -> zero_cells -> bin (wrap + cell + slot) -> scan(cell counts)
+//   wl_fill -> zero_cells -> bin (wrap + cell + slot) -> scan(cell counts)
 //   -> scatter -> cellsort (deterministic order inside a cell) -> permute
-//   -> copyback -> count (neighbours per atom, tile widths) -> scan(tile
-//   sizes) -> fill
+//   -> copyback -> nbr (search + list, one pass)
 //
 // The pair test is the reference's: on wrapped positions,
 //   d = x(i) - x(j);  d = d + L - L*int(1.5 + d/L);  r2 <= atrskin^2
@@ -25,8 +24,6 @@
 namespace fcmd {
 
 constexpr int WL_GRID = 2048;  // blocks of the work-list kernels (8 per CU)
-
-enum { WL_BLK = 0, WL_CCHK = 1, WL_TCHK = 2, WL_TRAJ = 3 };
 
 __device__ __forceinline__ bool rebuilding(const TrajScalars &ts) { return ts.rebuild && !ts.err; }
 
@@ -57,69 +54,79 @@ __device__ __forceinline__ long long block_excl_scan(long long v, long long *lds
     return off + inc - v;
 }
 
-// one block: where each rebuilding trajectory's items go in the work lists
-__global__ __launch_bounds__(BLK) void k_wl_scan(DevPtrs d, int ntraj)
+// One block per trajectory.  Each block sums the item counts of the
+// rebuilding trajectories before its own (all blocks repeat this small scan
+// over the trajectory flags instead of waiting for a separate launch), then
+// writes its trajectory's items at that offset; block 0 writes the lengths.
+__global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
 {
-    __shared__ long long lds[4];
-    long long carry[4] = {0, 0, 0, 0};
+    __shared__ int lds[4][2 * WL_LISTS];
+    const int t = blockIdx.x;
+    int below[WL_LISTS] = {}, total[WL_LISTS] = {};
     for (int base = 0; base < ntraj; base += BLK) {
-        const int t = base + threadIdx.x;
-        const bool f = t < ntraj && rebuilding(d.ts[t]);
-        const TrajParams *tp = f ? d.tp + t : nullptr;
-        const long long n[4] = {f ? tp->nblk : 0, f ? tp->ncblk : 0, f ? tp->ntblk : 0, f ? 1 : 0};
-        for (int k = 0; k < 4; ++k) {
-            long long tot;
-            const long long pre = block_excl_scan(n[k], lds, &tot) + carry[k];
-            if (t < ntraj) d.wl_off[4 * t + k] = f ? (int)pre : -1;
-            carry[k] += tot;
+        const int u = base + threadIdx.x;
+        const bool f = u < ntraj && rebuilding(d.ts[u]);
+        for (int k = 0; k < WL_LISTS; ++k) {
+            const int v = f ? d.wl_size[WL_LISTS * u + k] : 0;
+            below[k] += u < t ? v : 0;
+            total[k] += v;
         }
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 4; ++k) d.wl_cnt[k] = (int)carry[k];
-}
-
-// one block per trajectory: write its items
-__global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d)
-{
-    const int t = blockIdx.x;
-    const int *off = d.wl_off + 4 * t;
-    if (off[0] < 0) return;
+    const int w = threadIdx.x >> 6;
+    for (int k = 0; k < WL_LISTS; ++k) {
+        below[k] = wave_sum_i(below[k]);
+        total[k] = wave_sum_i(total[k]);
+        if ((threadIdx.x & 63) == 0) {
+            lds[w][k] = below[k];
+            lds[w][WL_LISTS + k] = total[k];
+        }
+    }
+    __syncthreads();
+    int off[WL_LISTS];
+    for (int k = 0; k < WL_LISTS; ++k) {
+        off[k] = lds[0][k] + lds[1][k] + lds[2][k] + lds[3][k];
+        if (t == 0 && threadIdx.x == 0)
+            d.wl_cnt[k] = lds[0][WL_LISTS + k] + lds[1][WL_LISTS + k] + lds[2][WL_LISTS + k] +
+                          lds[3][WL_LISTS + k];
+    }
+    if (!rebuilding(d.ts[t])) return;
     const TrajParams &tp = d.tp[t];
-    for (int q = threadIdx.x; q < tp.nblk; q += BLK) d.wl_blk[off[0] + q] = tp.bfirst + q;
-    for (int q = threadIdx.x; q < tp.ncblk; q += BLK) d.wl_cchk[off[1] + q] = tp.cfirst + q;
-    for (int q = threadIdx.x; q < tp.ntblk; q += BLK) d.wl_tchk[off[2] + q] = tp.tfirst + q;
-    if (threadIdx.x == 0) d.wl_traj[off[3]] = t;
+    for (int q = threadIdx.x; q < tp.nblk; q += BLK) d.wl_blk[off[WL_BLK] + q] = tp.bfirst + q;
+    for (int q = threadIdx.x; q < tp.ncblk; q += BLK) d.wl_cchk[off[WL_CCHK] + q] = tp.cfirst + q;
+    if (threadIdx.x == 0) d.wl_traj[off[WL_TRAJ]] = t;
 }
 
-// grid-stride loop over one work list; the body sees a uniform item per block
-#define FOR_ITEMS(list, k, item)                                                    \
-    for (int w_ = blockIdx.x, n_ = d.wl_cnt[k]; w_ < n_; w_ += gridDim.x)          \
+// Grid-stride loop over one work list, XCD-aware: the items are split into
+// NXCD contiguous runs, one per XCD (workgroup IDs go to the XCDs
+// round-robin), so the blocks one XCD runs are neighbours in space.
+// gridDim.x must be a multiple of NXCD.  The body sees a uniform item per block.
+#define FOR_ITEMS(list, k, item)                                                     \
+    for (int n_ = d.wl_cnt[k], c_ = (n_ + NXCD - 1) / NXCD,                           \
+             e_ = min(n_, (int)(blockIdx.x % NXCD + 1) * c_),                         \
+             w_ = (int)(blockIdx.x % NXCD) * c_ + (int)(blockIdx.x / NXCD);          \
+         w_ < e_; w_ += gridDim.x / NXCD)                                             \
         if (const int item = (list)[w_]; true)
 
 // ---------------------------------------------------------------------------
-// segmented exclusive scans over SCAN_CHUNK-element chunks.
-// kind 0: cells (in = cell_cnt, out = cell_start); kind 1: tiles (in = tile_w
-// scaled by TILE, out = tile_off)
-template <int KIND>
+// cell_start = exclusive scan of cell_cnt within each trajectory: a scan of
+// each SCAN_CHUNK-cell chunk, a scan of each trajectory's chunk totals, and
+// the add of the chunk offsets
 __global__ __launch_bounds__(BLK) void k_scan_chunk(DevPtrs d)
 {
     __shared__ long long lds[4];
-    FOR_ITEMS((KIND == 0 ? d.wl_cchk : d.wl_tchk), KIND == 0 ? WL_CCHK : WL_TCHK, c)
+    FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
         const int base = c * SCAN_CHUNK + threadIdx.x * 4;
-        long long v[4];
+        int v[4];
         long long s = 0;
         for (int k = 0; k < 4; ++k) {
-            v[k] = KIND == 0 ? (long long)d.cell_cnt[base + k] : (long long)d.tile_w[base + k] * TILE;
+            v[k] = d.cell_cnt[base + k];
             s += v[k];
         }
         long long total;
         long long pre = block_excl_scan(s, lds, &total);
         for (int k = 0; k < 4; ++k) {
-            if (KIND == 0)
-                d.cell_start[base + k] = (int)pre;
-            else
-                d.tile_off[base + k] = pre;
+            d.cell_start[base + k] = (int)pre;
             pre += v[k];
         }
         if (threadIdx.x == 0) d.scan_tmp_ll[c] = total;
@@ -127,16 +134,14 @@ __global__ __launch_bounds__(BLK) void k_scan_chunk(DevPtrs d)
 }
 
 // one block per rebuilding trajectory: exclusive scan of its chunk totals
-template <int KIND>
 __global__ __launch_bounds__(BLK) void k_scan_totals(DevPtrs d)
 {
     __shared__ long long lds[4];
     FOR_ITEMS(d.wl_traj, WL_TRAJ, t)
     {
         const TrajParams &tp = d.tp[t];
-        TrajScalars &ts = d.ts[t];
-        const int first = KIND == 0 ? tp.cfirst : tp.tfirst;
-        const int n = KIND == 0 ? tp.ncblk : tp.ntblk;
+        const int first = tp.cfirst;
+        const int n = tp.ncblk;
         long long carry = 0;
         for (int b0 = 0; b0 < n; b0 += 4 * BLK) {
             long long v[4];
@@ -156,28 +161,17 @@ __global__ __launch_bounds__(BLK) void k_scan_totals(DevPtrs d)
             }
             carry += total;
         }
-        if (KIND == 1 && threadIdx.x == 0) {
-            ts.nlist = (int)carry;
-            if (carry > tp.lcap) ts.err = ERR_LISTFULL;
-        }
         __syncthreads();
     }
 }
 
-template <int KIND>
 __global__ __launch_bounds__(BLK) void k_scan_add(DevPtrs d)
 {
-    FOR_ITEMS((KIND == 0 ? d.wl_cchk : d.wl_tchk), KIND == 0 ? WL_CCHK : WL_TCHK, c)
+    FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
-        const int t = KIND == 0 ? d.cblk_traj[c] : d.tblk_traj[c];
-        const long long add = d.scan_tmp_ll[c] + (KIND == 1 ? d.tp[t].lbase : 0);
+        const int add = (int)d.scan_tmp_ll[c];
         const int base = c * SCAN_CHUNK + threadIdx.x * 4;
-        for (int k = 0; k < 4; ++k) {
-            if (KIND == 0)
-                d.cell_start[base + k] += (int)add;
-            else
-                d.tile_off[base + k] += add;
-        }
+        for (int k = 0; k < 4; ++k) d.cell_start[base + k] += add;
     }
 }
 
@@ -322,7 +316,15 @@ __global__ __launch_bounds__(BLK) void k_copyback(DevPtrs d)
 // ---------------------------------------------------------------------------
 // The candidate search: every atom j != i of the 27 (or fewer, without
 // duplicates) cells around i's cell whose reference r2 is within the list
-// radius, in a fixed order; sink(j, cnt) is called for each.
+// radius, in a fixed order (cells z-major, then y, then x; atoms of one cell
+// by reference number); sink(j, cnt) is called for each.
+//
+// Atoms are sorted by cell, so the x-neighbouring cells of one (y, z) row
+// are one contiguous slot range, or two where the row wraps round the box.
+// The bounds of a z-plane's rows are loaded together before its first
+// candidate; each row is then walked as one sequence (its second range
+// following the first), CU candidates per iteration with their loads
+// issued together.  The visiting order is the same as cell by cell.
 template <typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                           int i, int &half, Sink sink)
@@ -339,32 +341,77 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     const int lx = ncx >= 3 ? -1 : 0, hx = ncx >= 3 ? 1 : ncx - 1;
     const int ly = ncy >= 3 ? -1 : 0, hy = ncy >= 3 ? 1 : ncy - 1;
     const int lz = ncz >= 3 ? -1 : 0, hz = ncz >= 3 ? 1 : ncz - 1;
+    // the x run cx+lx .. cx+hx (cyclic) as runs [xa0, xa1] then [xb0, xb1]
+    int xa0 = cx + lx, xa1 = cx + hx, xb0 = 0, xb1 = -1;
+    if (xa0 < 0) {
+        xa0 = xa0 + ncx;
+        xb1 = xa1;
+        xa1 = ncx - 1;
+    } else if (xa1 >= ncx) {
+        xb1 = xa1 - ncx;
+        xa1 = ncx - 1;
+    }
+    const int *cstart = d.cell_start + tp.cbase;
+    const int *ccnt = d.cell_cnt + tp.cbase;
+    constexpr int CU = 2;  // candidates whose loads are issued together
     int cnt = 0;
+#pragma unroll 1
     for (int oz = lz; oz <= hz; ++oz) {
         int zz = cz + oz;
         zz += (zz < 0) ? ncz : (zz >= ncz ? -ncz : 0);
-        for (int oy = ly; oy <= hy; ++oy) {
-            int yy = cy + oy;
-            yy += (yy < 0) ? ncy : (yy >= ncy ? -ncy : 0);
-            const int rowbase = (zz * ncy + yy) * ncx;
-            for (int ox = lx; ox <= hx; ++ox) {
-                int xx = cx + ox;
-                xx += (xx < 0) ? ncx : (xx >= ncx ? -ncx : 0);
-                const int cell = tp.cbase + rowbase + xx;
-                const int j0 = tp.abase + d.cell_start[cell];
-                const int j1 = j0 + d.cell_cnt[cell];
-                for (int j = j0; j < j1; ++j) {
+        // per row: first range [a0, a0 + la), then [b0, b0 + (len - la))
+        int a0[3], la[3], b0[3], len[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int oy = ly + q;
+            a0[q] = la[q] = b0[q] = len[q] = 0;
+            if (oy <= hy) {
+                int yy = cy + oy;
+                yy += (yy < 0) ? ncy : (yy >= ncy ? -ncy : 0);
+                const int row = (zz * ncy + yy) * ncx;
+                a0[q] = cstart[row + xa0];
+                la[q] = cstart[row + xa1] + ccnt[row + xa1] - a0[q];
+                len[q] = la[q];
+                if (xb1 >= xb0) {
+                    b0[q] = cstart[row + xb0];
+                    len[q] += cstart[row + xb1] + ccnt[row + xb1] - b0[q];
+                }
+            }
+        }
+        // rows one at a time (not unrolled: the row's bounds are shifted into
+        // slot 0, so the arrays stay in registers)
+#pragma unroll 1
+        for (int q = 0; q < 3; ++q) {
+            const int n = len[0], na = la[0];
+            const int base_a = tp.abase + a0[0], base_b = tp.abase + b0[0] - na;
+            a0[0] = a0[1], la[0] = la[1], b0[0] = b0[1], len[0] = len[1];
+            a0[1] = a0[2], la[1] = la[2], b0[1] = b0[2], len[1] = len[2];
+            for (int k0 = 0; k0 < n; k0 += CU) {
+                int jj[CU], oj[CU];
+                double xj[CU], yj[CU], zj[CU];
+#pragma unroll
+                for (int u = 0; u < CU; ++u) {
+                    const int k = min(k0 + u, n - 1);
+                    jj[u] = k < na ? base_a + k : base_b + k;
+                    oj[u] = d.oid[jj[u]];
+                    xj[u] = d.x[jj[u]];
+                    yj[u] = d.y[jj[u]];
+                    zj[u] = d.z[jj[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < CU; ++u) {
+                    if (k0 + u >= n) break;
+                    const int j = jj[u];
                     if (j == i) continue;
-                    const int oj = d.oid[j];
                     double dx, dy, dz;
-                    if (oi < oj) {
-                        dx = xi - d.x[j];
-                        dy = yi - d.y[j];
-                        dz = zi - d.z[j];
+                    if (oi < oj[u]) {
+                        dx = xi - xj[u];
+                        dy = yi - yj[u];
+                        dz = zi - zj[u];
                     } else {
-                        dx = d.x[j] - xi;
-                        dy = d.y[j] - yi;
-                        dz = d.z[j] - zi;
+                        dx = xj[u] - xi;
+                        dy = yj[u] - yi;
+                        dz = zj[u] - zi;
                     }
                     dx = dx + bx - bx * fint(1.5 + dx * ibx);
                     dy = dy + by - by * fint(1.5 + dy * iby);
@@ -373,7 +420,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                     if (r2 <= rs2) {
                         sink(j, cnt);
                         ++cnt;
-                        half += (oi < oj);
+                        half += (oi < oj[u]);
                     }
                 }
             }
@@ -382,15 +429,12 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     return cnt;
 }
 
-// scratch slot of candidate k of atom slot i (tile-major, coalesced per wave)
-__device__ __forceinline__ size_t tmp_index(int i, int k)
-{
-    return ((size_t)(i / TILE) * NB_TMP + k) * TILE + (i & (TILE - 1));
-}
-
-// count: neighbours per atom, tile widths, pair count; the first NB_TMP
-// neighbours of each atom are kept so that the fill only copies them
-__global__ __launch_bounds__(BLK) void k_nbr_count(DevPtrs d)
+// The list, in one pass: search, write each atom's neighbours to its lwidth
+// slots, count the pairs and the widest atom.  An atom with more than lwidth
+// neighbours stops the trajectory with LISTFULL (the reference stops when its
+// list array is nearly full, neighbor-pbc.f:89-93); its count is clamped so
+// that no later kernel reads past its slots.
+__global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
 {
     __shared__ int ldsi[4];
     FOR_ITEMS(d.wl_blk, WL_BLK, b)
@@ -400,42 +444,23 @@ __global__ __launch_bounds__(BLK) void k_nbr_count(DevPtrs d)
         TrajScalars &ts = d.ts[t];
         const int i = b * BLK + threadIdx.x;
         const int rel = i - tp.abase;
+        const int width = tp.lwidth;
         int cnt = 0, half = 0;
         if (rel < tp.natom) {
-            int *tmp = d.nbtmp;
+            int *out = d.list + list_slot(tp, rel);
             cnt = candidates(d, tp, ts, i, half, [&](int j, int k) {
-                if (k < NB_TMP) tmp[tmp_index(i, k)] = j;
+                if (k < width) out[(size_t)k * TILE] = j;
             });
+        }
+        if (cnt > width) {
+            ts.err = ERR_LISTFULL;
+            cnt = width;
         }
         d.nn[i] = cnt;
         const int w = wave_max_i(cnt);
-        if ((threadIdx.x & (TILE - 1)) == 0) d.tile_w[tp.tbase + rel / TILE] = w;
+        if ((threadIdx.x & (TILE - 1)) == 0 && w > 0) atomicMax(&ts.nlist, w);
         const int h = block_sum_i(half, ldsi);
         if (threadIdx.x == 0 && h) atomicAdd(&ts.npairs, h);
-    }
-}
-
-// fill the sliced-ELL list: copy the kept neighbours, search again only for
-// atoms with more than NB_TMP (same order, so the same list)
-__global__ __launch_bounds__(BLK) void k_nbr_fill(DevPtrs d)
-{
-    FOR_ITEMS(d.wl_blk, WL_BLK, b)
-    {
-        const int t = d.blk_traj[b];
-        const TrajParams &tp = d.tp[t];
-        const TrajScalars &ts = d.ts[t];
-        if (ts.err) continue;  // list overflow found by the tile scan
-        const int i = b * BLK + threadIdx.x;
-        const int rel = i - tp.abase;
-        if (rel >= tp.natom) continue;
-        const int nn = d.nn[i];
-        int *out = d.list + d.tile_off[tp.tbase + rel / TILE] + (i & (TILE - 1));
-        if (nn <= NB_TMP) {
-            for (int k = 0; k < nn; ++k) out[(size_t)k * TILE] = d.nbtmp[tmp_index(i, k)];
-        } else {
-            int half = 0;
-            candidates(d, tp, ts, i, half, [&](int j, int k) { out[(size_t)k * TILE] = j; });
-        }
     }
 }
 
@@ -443,15 +468,14 @@ __global__ __launch_bounds__(BLK) void k_nbr_fill(DevPtrs d)
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
     const dim3 blk(BLK);
-    const dim3 ga(std::min(g.nblk, WL_GRID)), gc(std::min(g.ncblk, WL_GRID));
-    const dim3 gt(std::min(g.ntblk, WL_GRID)), gj(std::min(g.ntraj, WL_GRID));
-    hipLaunchKernelGGL(k_wl_scan, dim3(1), blk, 0, s, d, g.ntraj);
-    hipLaunchKernelGGL(k_wl_fill, dim3(g.ntraj), blk, 0, s, d);
+    const dim3 ga(xcd_grid(std::min(g.nblk, WL_GRID))), gc(xcd_grid(std::min(g.ncblk, WL_GRID)));
+    const dim3 gj(xcd_grid(std::min(g.ntraj, WL_GRID)));
+    hipLaunchKernelGGL(k_wl_fill, dim3(g.ntraj), blk, 0, s, d, g.ntraj);
     hipLaunchKernelGGL(k_zero_cells, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_bin, ga, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_chunk<0>, gc, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_totals<0>, gj, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_add<0>, gc, blk, 0, s, d);
+    hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
+    hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
+    hipLaunchKernelGGL(k_scan_add, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, ga, blk, 0, s, d);
     hipLaunchKernelGGL(k_cellsort, gc, blk, 0, s, d);
     if (mixed) {
@@ -461,11 +485,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
         hipLaunchKernelGGL(k_permute<double>, ga, blk, 0, s, d);
         hipLaunchKernelGGL(k_copyback<double>, ga, blk, 0, s, d);
     }
-    hipLaunchKernelGGL(k_nbr_count, ga, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_chunk<1>, gt, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_totals<1>, gj, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_add<1>, gt, blk, 0, s, d);
-    hipLaunchKernelGGL(k_nbr_fill, ga, blk, 0, s, d);
+    hipLaunchKernelGGL(k_nbr, ga, blk, 0, s, d);
 }
 
 }  // namespace fcmd

@@ -74,6 +74,7 @@ __device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *ste
     ts.dmms = 0.0;
     ts.dmmst = 0;
     ts.npairs = 0;
+    ts.nlist = 0;
     ts.rebuild = 1;
     ts.nrebuild += 1;
 }
@@ -164,7 +165,8 @@ template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
 {
     __shared__ double lds[4];
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     TrajScalars &ts = d.ts[t];
@@ -342,9 +344,7 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
     const double bf2 = bf1 + 0.5;
     const double rc2 = tp.atrcut * tp.atrcut;
     const int nn = d.nn[i];
-    const int lane = threadIdx.x & (TILE - 1);
-    const int tile = tp.tbase + rel / TILE;
-    const int *lp = d.list + d.tile_off[tile] + lane;
+    const int *lp = d.list + list_slot(tp, rel);
     KahanF kx, ky, kz;
     const float rc2f = (float)rc2;
     fx = fy = fz = 0.0;
@@ -427,7 +427,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
     __shared__ double lds[4];
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     const TrajScalars &ts = d.ts[t];
@@ -502,7 +503,8 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_kick_nh(DevPtrs d)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     const TrajScalars &ts = d.ts[t];
@@ -537,19 +539,29 @@ void launch_neighbor_entry(const DevPtrs &d, int traj, int ntraj, int step, int 
     hipLaunchKernelGGL(k_neighbor_entry, dim3(1), dim3(64), 0, s, d, traj, ntraj, step, step_out);
 }
 
-void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
+void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
     if (mixed)
-        hipLaunchKernelGGL(k_drift<float>, dim3(g.nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_drift<float>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     else
-        hipLaunchKernelGGL(k_drift<double>, dim3(g.nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_drift<double>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+}
+
+void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s)
+{
     hipLaunchKernelGGL(k_traj_pre, dim3(g.ntraj), dim3(BLK), 0, s, d);
 }
 
-void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                  hipStream_t s)
+void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
-    const dim3 grid(g.nblk), blk(BLK);
+    launch_drift_atoms(d, g, mixed, s);
+    launch_traj_pre(d, g, s);
+}
+
+void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
+                        hipStream_t s)
+{
+    const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
     if (mixed) {
         if (single_species)
             hipLaunchKernelGGL((k_force<float, false>), grid, blk, 0, s, d, mode);
@@ -561,7 +573,18 @@ void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single
         else
             hipLaunchKernelGGL((k_force<double, true>), grid, blk, 0, s, d, mode);
     }
+}
+
+void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+{
     hipLaunchKernelGGL(k_traj_post, dim3(g.ntraj), dim3(BLK), 0, s, d, mode);
+}
+
+void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
+                  hipStream_t s)
+{
+    launch_force_atoms(d, g, mixed, single_species, mode, s);
+    launch_traj_post(d, g, mode, s);
 }
 
 __global__ void k_clear_pending(DevPtrs d, int ntraj)
@@ -573,9 +596,9 @@ __global__ void k_clear_pending(DevPtrs d, int ntraj)
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
     if (mixed)
-        hipLaunchKernelGGL(k_kick_nh<float>, dim3(g.nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_kick_nh<float>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     else
-        hipLaunchKernelGGL(k_kick_nh<double>, dim3(g.nblk), dim3(BLK), 0, s, d);
+        hipLaunchKernelGGL(k_kick_nh<double>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     hipLaunchKernelGGL(k_clear_pending, dim3((g.ntraj + 255) / 256), dim3(256), 0, s, d, g.ntraj);
 }
 

@@ -79,6 +79,9 @@ _PD = C.POINTER(C.c_double)
 _PI = C.POINTER(C.c_int)
 _lib = None
 
+# fcmd_time_kernels entries, in the order of include/fcmd.h (FCMD_NTIMED)
+TIMED_KERNELS = ("drift", "traj_pre", "rebuild", "force", "traj_post", "kick")
+
 EXPORTS = ("fcmd_device_count", "fcmd_error_text", "fcmd_create", "fcmd_destroy",
            "fcmd_set_params", "fcmd_upload", "fcmd_download", "fcmd_set_scalars",
            "fcmd_get_scalars", "fcmd_neighbor", "fcmd_accatom", "fcmd_integrator",
@@ -292,9 +295,9 @@ class System:
 
     def time_kernels(self, nsteps: int) -> dict:
         """Mean ms per step of each kernel class (HIP events, engine stream)."""
-        out = np.zeros(4)
+        out = np.zeros(len(TIMED_KERNELS))
         check(lib().fcmd_time_kernels(self.h, int(nsteps), _p(out, _PD)))
-        return dict(zip(("drift", "rebuild", "force", "kick"), out.tolist()))
+        return dict(zip(TIMED_KERNELS, out.tolist()))
 
     def list_entries(self) -> int:
         v = C.c_longlong()

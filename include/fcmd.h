@@ -73,7 +73,8 @@ typedef struct {
 /* Scalars of one trajectory (ATOMP atrskin, psi, s; ENERG ekin, epot,
  * sumv2; DMMPP), plus engine counters: dmmsr is dmms as it stood when the
  * last rebuild was decided (what neighbor-pbc-mav.f:232 prints), npairs the
- * pairs in the current list, nrebuild the rebuilds so far. */
+ * pairs in the current list, nrebuild the rebuilds so far, nlist the largest
+ * neighbour count of any atom in the current list (at most list_width). */
 typedef struct {
     double atrskin, psi, s, ekin, epot, sumv2;
     double dmmh[256];
@@ -86,7 +87,11 @@ const char *fcmd_error_text(int code);
 
 /* Allocate an ensemble of ntraj trajectories on the current HIP device.
  * precision: FCMD_FP64 or FCMD_MIXED.  list_width: neighbour-list slots per
- * atom reserved (the role of const.inc mxnlist); 0 selects 128. */
+ * atom, i.e. the most neighbours (within atrskin, both directions) any atom
+ * may have; 0 selects 128.  It takes the role of const.inc mxnlist: a list
+ * rebuild that finds an atom with more neighbours fails with
+ * FCMD_ERR_LISTFULL, as the reference stops when its list array is nearly
+ * full (neighbor-pbc.f:89-93).  Memory: 4 * list_width bytes per atom. */
 int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int precision,
                 int list_width);
 void fcmd_destroy(fcmd_system *s);
@@ -160,8 +165,11 @@ int fcmd_init_random(unsigned long long seed, int natom, const double *box, doub
 int fcmd_last_run_ms(fcmd_system *s, double *total_ms);
 
 /* Run nsteps steps (no graph) with HIP events on the engine stream around
- * each kernel class; avg_ms[4] receives the mean milliseconds per step of
- * {drift, list-rebuild sequence, force, Nose-Hoover kick}. */
+ * each launch; avg_ms[FCMD_NTIMED] receives the mean milliseconds per step of
+ * {k_drift, k_traj_pre, list-rebuild sequence, k_force, k_traj_post,
+ * Nose-Hoover kick}.  The kick is deferred to the next drift inside a run, so
+ * it is timed once over the nsteps and its entry is that time / nsteps. */
+#define FCMD_NTIMED 6
 int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms);
 
 /* Full-list entries (2 x pairs) summed over all trajectories' current lists. */

@@ -33,8 +33,8 @@ def acc_err(got, ref):
     return max(rel_err(got[k], ref[k]) for k in ("ax", "ay", "az"))
 
 
-def gpu_system(case, variant, precision=fcmd.FP64):
-    s = fcmd.System(fcmd.make_params(case, variant), precision)
+def gpu_system(case, variant, precision=fcmd.FP64, list_width=0):
+    s = fcmd.System(fcmd.make_params(case, variant), precision, list_width=list_width)
     s.upload(case)
     s.set_scalars(psi=float(case["psi"]), s=float(case["s"]), dmmh=case["dmmh"],
                   dmms=float(case["dmms"]), dmmsu=float(case["dmmsu"]), dmmax=float(case["dmmax"]),
@@ -171,6 +171,33 @@ def test_larger_system_one_step(n, rho, seed, variant):
     out = s.stoddard(1)
     np.testing.assert_array_equal(out["cl"], o2.arr["cl"])
     np.testing.assert_array_equal(out["csi"][1:], o2.arr["csi"][1:])
+
+
+def test_list_width_is_the_widest_atom():
+    """list_width bounds the neighbours of one atom: the rebuild reports the
+    widest atom (nlist, against the oracle's list), runs with list_width equal
+    to it, and one below stops with the reference's list-full error."""
+    case = cases.make_case(n=2000, rho=0.04, seed=35, dense_frac=0.5, ncenters=3)
+    o = po.Oracle(case, "mav")
+    o.neighbor(0)
+    deg = np.zeros(case["natom"] + 1, dtype=np.int64)
+    for i, j in o.pairs():
+        deg[i] += 1
+        deg[j] += 1
+    widest = int(deg.max())
+    assert widest > 8
+    s = gpu_system(case, "mav")
+    s.neighbor(0)
+    assert s.scalars().nlist == widest
+    s = gpu_system(case, "mav", list_width=widest)
+    s.neighbor(0)
+    lst, idx = s.halflist()
+    ref_l, ref_i = o.halflist()
+    np.testing.assert_array_equal(lst, ref_l)
+    s = gpu_system(case, "mav", list_width=widest - 1)
+    with pytest.raises(fcmd.FcmdError, match="Neighborlist is nearly full") as e:
+        s.neighbor(0)
+    assert e.value.code == 2
 
 
 def test_ensemble_matches_single_runs():
