@@ -339,6 +339,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.x2, N) A(d.y2, N) A(d.z2, N) A(d.vx2, N) A(d.vy2, N) A(d.vz2, N);
     if (s->mixed) {
         A(d.axf, N) A(d.ayf, N) A(d.azf, N) A(d.axf2, N) A(d.ayf2, N) A(d.azf2, N);
+        A(d.posf, N);
     } else {
         A(d.ax, N) A(d.ay, N) A(d.az, N) A(d.ax2, N) A(d.ay2, N) A(d.az2, N);
     }
@@ -454,6 +455,8 @@ int fcmd_upload(fcmd_system *s, int traj, const double *x, const double *y, cons
         HIPCHK(hipMemcpyAsync(s->stage_i, api, sizeof(int) * n, hipMemcpyHostToDevice, s->stream));
         hipLaunchKernelGGL(k_scatter_typ, grid, blk, 0, s->stream, s->d.typ, s->stage_i, s->d.inv, t.abase, n);
     }
+    if (x || y || z) launch_refresh_posf(s->d, traj, n, s->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     return FCMD_OK;
 }

@@ -11,6 +11,29 @@ namespace fcmd {
 // to sigma) satisfies by far.
 __device__ __forceinline__ double fint(double v) { return trunc(v); }
 
+// Mixed-path position record of one atom: its offset from the centre of the
+// cell it was binned into at the last list build, in FP32 (at most half a cell
+// side plus the drift since, so rounding costs ~1e-7 sigma), and the cell's
+// coordinates packed CELL_BITS each in w.  A pair separation then needs no
+// FP64: (cell_i - cell_j) * side + (off_i - off_j) with the cell difference
+// wrapped to a neighbour.
+__device__ __forceinline__ float4 make_posf(double x, double y, double z, int cx, int cy, int cz,
+                                            const TrajScalars &ts)
+{
+    return make_float4((float)(x - ((double)cx + 0.5) * ts.cside[0]),
+                       (float)(y - ((double)cy + 0.5) * ts.cside[1]),
+                       (float)(z - ((double)cz + 0.5) * ts.cside[2]),
+                       __int_as_float(cx | (cy << CELL_BITS) | (cz << (2 * CELL_BITS))));
+}
+
+__device__ __forceinline__ void unpack_cell(float w, int &cx, int &cy, int &cz)
+{
+    const int c = __float_as_int(w);
+    cx = c & CELL_MAX;
+    cy = (c >> CELL_BITS) & CELL_MAX;
+    cz = c >> (2 * CELL_BITS);
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

@@ -25,6 +25,8 @@ constexpr int MAXS = 10;    // species, const.inc:12 mxsorts
 constexpr int EMAX = 48;    // cluster edges (r <= atskin) kept per atom
 constexpr int NPART = 4;    // per-block partial sums
 constexpr int NXCD = 8;     // XCDs of one MI355X, each with its own L2
+constexpr int CELL_BITS = 10;                   // per coordinate in a packed cell
+constexpr int CELL_MAX = (1 << CELL_BITS) - 1;  // cells per box side at most
 
 // XCD-aware block order.  Workgroup IDs are handed to the XCDs round-robin;
 // logical block (p % NXCD) * chunk + p / NXCD for workgroup p gives every XCD
@@ -134,6 +136,10 @@ struct DevPtrs {
     // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE]
     int *nn;
     int *list;
+    // mixed path only (null otherwise): per slot, the FP32 offset of the atom
+    // from the centre of its cell at the last list build, and that cell packed
+    // in w (make_posf)
+    float4 *posf;
     // clusters: per slot
     ClusterScalars *cls;
     int *edeg, *edeg1, *edges, *parent, *parentf, *liq;

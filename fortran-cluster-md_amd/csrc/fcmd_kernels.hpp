@@ -29,6 +29,8 @@ void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream
 // applies any pending Nose-Hoover final update and clears the flags
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
+// after positions of trajectory t are replaced from the host (mixed path)
+void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s);
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s);
 
 }  // namespace fcmd

@@ -256,14 +256,24 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
     }
 }
 
+// gather every per-atom array into cell order (into the alternate buffers);
+// on the mixed path also write each atom's cell-offset record for its cell
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 {
     FOR_ITEMS(d.wl_blk, WL_BLK, b)
     {
-        const TrajParams &tp = d.tp[d.blk_traj[b]];
+        const int t = d.blk_traj[b];
+        const TrajParams &tp = d.tp[t];
         const int i = b * BLK + threadIdx.x;
-        const int o = (i - tp.abase < tp.natom) ? d.sorted[i] : i;
+        const bool act = i - tp.abase < tp.natom;
+        const int o = act ? d.sorted[i] : i;
+        if (sizeof(AccT) == 4 && act) {
+            const TrajScalars &ts = d.ts[t];
+            const int c = d.atom_cell[o];
+            const int cx = c % ts.ncx, cy = (c / ts.ncx) % ts.ncy, cz = c / (ts.ncx * ts.ncy);
+            d.posf[i] = make_posf(d.x[o], d.y[o], d.z[o], cx, cy, cz, ts);
+        }
         d.x2[i] = d.x[o];
         d.y2[i] = d.y[o];
         d.z2[i] = d.z[o];

@@ -20,13 +20,14 @@ __device__ __forceinline__ bool nh_coupled(const TrajParams &tp, int typ)
 }
 
 // Cell grid for a list radius: side >= atrskin*(1+1e-9), so every pair within
-// atrskin lies in neighbouring cells even after rounding of x/side.
+// atrskin lies in neighbouring cells even after rounding of x/side; at most
+// CELL_MAX cells per side (wider cells only add candidates).
 __device__ void set_cell_grid(const TrajParams &tp, TrajScalars &ts)
 {
     int n[3];
     for (int k = 0; k < 3; ++k) {
         n[k] = (int)floor(tp.box[k] / (ts.atrskin * (1.0 + 1e-9)));
-        if (n[k] < 1) n[k] = 1;
+        n[k] = n[k] < 1 ? 1 : (n[k] > CELL_MAX ? CELL_MAX : n[k]);
         ts.cside[k] = tp.box[k] / (double)n[k];
     }
     if ((long long)n[0] * n[1] * n[2] > (long long)tp.ncpad) {
@@ -212,9 +213,15 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
             vy = vy + 0.5 * ay * dt;
             vz = vz + 0.5 * az * dt;
         }
-        d.x[i] = d.x[i] + dx;
-        d.y[i] = d.y[i] + dy;
-        d.z[i] = d.z[i] + dz;
+        const double xn = d.x[i] + dx, yn = d.y[i] + dy, zn = d.z[i] + dz;
+        d.x[i] = xn;
+        d.y[i] = yn;
+        d.z[i] = zn;
+        if (sizeof(AccT) == 4) {
+            int cx, cy, cz;
+            unpack_cell(d.posf[i].w, cx, cy, cz);
+            d.posf[i] = make_posf(xn, yn, zn, cx, cy, cz, ts);
+        }
         d.vx[i] = vx;
         d.vy[i] = vy;
         d.vz[i] = vz;
@@ -327,16 +334,18 @@ __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
 // Pair sum of one atom over its full-list neighbours.  FORM = ACC_1:
 // for-a-lj-n-pbc.f:95-105 (sum of r^-2 r^-6 (c2 r^-6 - c3) d, scaled by
 // 48 eps / m afterwards); FORM = ACC_M: for-a-lj-n-pbc-m.f:377-400 (per-pair
-// 48 eps_ab r^-2 r^-6 (c2_ab r^-6 - c3_ab) d).  AccT = double: FP64
-// throughout.  AccT = float: the minimum-image difference in FP64, the pair
-// distance and LJ terms in FP32, forces summed in FP32 with Kahan
-// compensation, energies summed in FP64.
-template <typename AccT, int FORM>
-__device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
-                                         int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                         double &e)
+// 48 eps_ab r^-2 r^-6 (c2_ab r^-6 - c3_ab) d).  Neighbours are taken U at a
+// time, all loads of a chunk issued before its arithmetic; pairs are summed
+// in list order.
+//
+// FP64: the reference's arithmetic, minimum image in its form, so every pair
+// term and cutoff decision is the reference's, bit for bit.
+template <int FORM>
+__device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
+                                              int i, int rel, int ti, double &fx, double &fy, double &fz,
+                                              double &e)
 {
-    constexpr bool MIXED = sizeof(AccT) == 4;
+    constexpr int U = 4;
     const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
     const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
@@ -345,16 +354,10 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
     const double rc2 = tp.atrcut * tp.atrcut;
     const int nn = d.nn[i];
     const int *lp = d.list + list_slot(tp, rel);
-    KahanF kx, ky, kz;
-    const float rc2f = (float)rc2;
     fx = fy = fz = 0.0;
-    // neighbours in chunks of four: all index and position loads of a chunk
-    // are issued before its arithmetic; pairs are still summed in list order
-    constexpr int U = 4;
     for (int s0 = 0; s0 < nn; s0 += U) {
-        int jj[U];
+        int jj[U], tj[U];
         double xj[U], yj[U], zj[U];
-        int tj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? lp[(size_t)(s0 + u) * TILE] : i;
 #pragma unroll
@@ -373,50 +376,111 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
             dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
             dy = dy + by * (bf1 - fint(bf2 + dy * iby));
             dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
-            const int pij = (FORM == ACC_1) ? 0 : ti * MAXS + tj[u];
-            if (!MIXED) {
-                double rij2 = dx * dx + dy * dy + dz * dz;
-                if (rij2 <= rc2) {
-                    rij2 = 1.0 / rij2;
-                    const double rij6 = rij2 * rij2 * rij2;
-                    double fij;
-                    if (FORM == ACC_1) {
-                        fij = rij2 * rij6 * (tp.c2[0] * rij6 - tp.c3[0]);
-                        e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
-                    } else {
-                        fij = tp.e48[pij] * rij2 * rij6 * (tp.c2[pij] * rij6 - tp.c3[pij]);
-                        e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
-                    }
-                    fx = fx + fij * dx;
-                    fy = fy + fij * dy;
-                    fz = fz + fij * dz;
+            double rij2 = dx * dx + dy * dy + dz * dz;
+            if (rij2 <= rc2) {
+                rij2 = 1.0 / rij2;
+                const double rij6 = rij2 * rij2 * rij2;
+                double fij;
+                if (FORM == ACC_1) {
+                    fij = rij2 * rij6 * (tp.c2[0] * rij6 - tp.c3[0]);
+                    e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
+                } else {
+                    const int pij = ti * MAXS + tj[u];
+                    fij = tp.e48[pij] * rij2 * rij6 * (tp.c2[pij] * rij6 - tp.c3[pij]);
+                    e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
                 }
-            } else {
-                const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
-                const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
-                if (r2 <= rc2f) {
-                    const float ri2 = 1.0f / r2;
-                    const float r6 = ri2 * ri2 * ri2;
-                    float fij;
-                    if (FORM == ACC_1) {
-                        fij = ri2 * r6 * (tp.c2f[0] * r6 - tp.c3f[0]);
-                        e += (double)(r6 * (tp.c2f[0] * r6 - tp.c1f[0]));
-                    } else {
-                        fij = (float)tp.e48[pij] * ri2 * r6 * (tp.c2f[pij] * r6 - tp.c3f[pij]);
-                        e += tp.e4[pij] * (double)(r6 * (tp.c2f[pij] * r6 - tp.c1f[pij]));
-                    }
-                    kx.add(fij * dxf);
-                    ky.add(fij * dyf);
-                    kz.add(fij * dzf);
-                }
+                fx = fx + fij * dx;
+                fy = fy + fij * dy;
+                fz = fz + fij * dz;
             }
         }
     }
-    if (MIXED) {
-        fx = (double)kx.s;
-        fy = (double)ky.s;
-        fz = (double)kz.s;
+}
+
+// one coordinate of a mixed-path separation: cell difference wrapped to a
+// neighbour (every listed pair was in neighbouring cells at the list build),
+// then, for a box of fewer than 3 cells, the remaining image by rounding
+__device__ __forceinline__ float sep_mixed(int ci, int cj, int nc, float side, float off, float box,
+                                           float ibox)
+{
+    int dc = ci - cj;
+    dc += dc > 1 ? -nc : (dc < -1 ? nc : 0);
+    float s = (float)dc * side + off;
+    if (nc < 3) s -= box * rintf(s * ibox);
+    return s;
+}
+
+// Mixed: separations from the FP32 cell-offset records (make_posf: one 16-byte
+// gather per neighbour, no FP64), pair distance and LJ terms in FP32, forces
+// summed in FP32 with Kahan compensation, energies summed in FP64.
+template <int FORM>
+__device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
+                                               int i, int rel, int ti, double &fx, double &fy, double &fz,
+                                               double &e)
+{
+    constexpr int U = 4;
+    const float4 pi = d.posf[i];
+    int cxi, cyi, czi;
+    unpack_cell(pi.w, cxi, cyi, czi);
+    const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
+    const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
+    const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
+    const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
+    const float rc2f = (float)(tp.atrcut * tp.atrcut);
+    const int nn = d.nn[i];
+    const int *lp = d.list + list_slot(tp, rel);
+    KahanF kx, ky, kz;
+    for (int s0 = 0; s0 < nn; s0 += U) {
+        int jj[U], tj[U];
+        float4 pj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? lp[(size_t)(s0 + u) * TILE] : i;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            pj[u] = d.posf[jj[u]];
+            tj[u] = (FORM == ACC_1) ? 0 : d.typ[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (s0 + u >= nn) break;
+            int cxj, cyj, czj;
+            unpack_cell(pj[u].w, cxj, cyj, czj);
+            const float dxf = sep_mixed(cxi, cxj, ncx, sx, pi.x - pj[u].x, bx, ibx);
+            const float dyf = sep_mixed(cyi, cyj, ncy, sy, pi.y - pj[u].y, by, iby);
+            const float dzf = sep_mixed(czi, czj, ncz, sz, pi.z - pj[u].z, bz, ibz);
+            const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
+            if (r2 <= rc2f) {
+                const float ri2 = 1.0f / r2;
+                const float r6 = ri2 * ri2 * ri2;
+                float fij;
+                if (FORM == ACC_1) {
+                    fij = ri2 * r6 * (tp.c2f[0] * r6 - tp.c3f[0]);
+                    e += (double)(r6 * (tp.c2f[0] * r6 - tp.c1f[0]));
+                } else {
+                    const int pij = ti * MAXS + tj[u];
+                    fij = (float)tp.e48[pij] * ri2 * r6 * (tp.c2f[pij] * r6 - tp.c3f[pij]);
+                    e += tp.e4[pij] * (double)(r6 * (tp.c2f[pij] * r6 - tp.c1f[pij]));
+                }
+                kx.add(fij * dxf);
+                ky.add(fij * dyf);
+                kz.add(fij * dzf);
+            }
+        }
     }
+    fx = (double)kx.s;
+    fy = (double)ky.s;
+    fz = (double)kz.s;
+}
+
+template <typename AccT, int FORM>
+__device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
+                                         int i, int rel, int ti, double &fx, double &fy, double &fz,
+                                         double &e)
+{
+    if constexpr (sizeof(AccT) == 4)
+        pair_sum_mixed<FORM>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+    else
+        pair_sum_fp64<FORM>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
 }
 
 // LJ forces over the full neighbour list plus the integrator's fused
@@ -518,6 +582,25 @@ __global__ __launch_bounds__(BLK) void k_kick_nh(DevPtrs d)
     d.vx[i] = (d.vx[i] + dth * ax) / den;
     d.vy[i] = (d.vy[i] + dth * ay) / den;
     d.vz[i] = (d.vz[i] + dth * az) / den;
+}
+
+// positions replaced from the host: recompute the mixed-path cell offsets of
+// trajectory t's atoms against the cells of its last list build
+__global__ __launch_bounds__(BLK) void k_refresh_posf(DevPtrs d, int t)
+{
+    const TrajParams &tp = d.tp[t];
+    const TrajScalars &ts = d.ts[t];
+    const int rel = blockIdx.x * BLK + threadIdx.x;
+    if (rel >= tp.natom) return;
+    const int i = tp.abase + rel;
+    int cx, cy, cz;
+    unpack_cell(d.posf[i].w, cx, cy, cz);
+    d.posf[i] = make_posf(d.x[i], d.y[i], d.z[i], cx, cy, cz, ts);
+}
+
+void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s)
+{
+    if (d.posf) hipLaunchKernelGGL(k_refresh_posf, dim3((natom + BLK - 1) / BLK), dim3(BLK), 0, s, d, t);
 }
 
 // neighbor(step) called directly (init-cube1.f:113, read-rst.f:63): only
