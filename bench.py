@@ -172,6 +172,9 @@ def main():
                     help="launch every step's kernels on the stream instead of replaying a graph")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (262k FP64, 2M FP64 vs mixed, mixture)")
+    ap.add_argument("--nbtim", type=int, default=0,
+                    help="override the deck's neighbour update time (nbtim, the adaptive skin's "
+                         "target rebuild interval); 0 keeps the workload's")
     args = ap.parse_args()
     use_graph = not args.no_graph
     if args.gpus != 1:
@@ -184,6 +187,9 @@ def main():
     K, Wm = args.steps, args.warmup
     log(f"[bench] building workload {args.workload}")
     params, states, meta = W.WORKLOADS[args.workload]()
+    if args.nbtim:
+        for p in params:
+            p.nbtim = args.nbtim
     natoms = int(sum(p.natom for p in params))
     every = meta.get("cluster_every", 0)
     fpt_size = meta.get("fpt_size", 50)
