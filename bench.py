@@ -17,6 +17,9 @@ Reported:
                    same state; energy-drift ratio mixed/FP64
   roofline         dominant kernel, HIP events on the engine stream
   cpu_baseline     the reference Fortran (flang -O3) on the host cores
+  dtype            f32+f64: pair arithmetic in FP32, state, integrator and
+                   reductions in FP64 (the fp64_* fields are FP64 throughout)
+  tuning           launch tunables the autotuner picked during the warmup
 
 Usage: python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
 """
@@ -87,6 +90,7 @@ def measure(name, precisions, steps, warmup=10, cluster_every=None, series_every
         with fcmd.System(params, prec) as s:
             W.setup(s, params, states)
             s.run(warmup, use_graph=True)
+            s.autotune(rounds=3)
             h0 = hamiltonian(s, params)
             t0 = time.perf_counter()
             series, _ = s.run(steps, cluster_every=series_every or every, use_graph=True)
@@ -200,6 +204,8 @@ def main():
         log(f"[bench] {name}: setup {len(params)} trajectories, {natoms} atoms")
         s = build_system(params, states, prec)
         s.run(Wm, cluster_every=0, use_graph=use_graph)
+        tuning = s.autotune(rounds=5)  # 30 more untimed steps
+        log(f"[bench] {name}: tuning {tuning}")
         h0 = hamiltonian(s, params)
         nr0 = np.array([s.scalars(t).nrebuild for t in range(len(params))])
         t0 = time.perf_counter()
@@ -214,7 +220,8 @@ def main():
         res[name] = dict(wall=wall, ev_ms=ev_ms, rate=natoms * K / wall,
                          nlist=s.list_entries(), largest=int(series.max()) if series.size else 0,
                          nfpt=int((fpt >= 0).sum()),
-                         rebuild_frac=float((sizes * (nr1 - nr0)).sum() / (natoms * K)))
+                         rebuild_frac=float((sizes * (nr1 - nr0)).sum() / (natoms * K)),
+                         tuning=tuning)
         log(f"[bench] {name}: {K} steps in {wall:.3f} s (events {ev_ms / 1e3:.3f} s) -> "
             f"{res[name]['rate']:.4e} atom-steps/s")
         if name == "mixed":
@@ -291,7 +298,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair math + fp64 state/reductions (mixed); fp64 for fp64_* fields",
+        "dtype": "f32+f64",
         "data": "synthetic: seeded LJ vapour (uniform, 0.9 sigma minimum, Maxwell-Boltzmann)",
         "config": {"workload": args.workload, "trajectories": len(params), "atoms": natoms,
                    "cluster_every": every, "fpt_size": fpt_size, "parallelism": "1 GPU"},
@@ -303,6 +310,7 @@ def main():
         "energy_drift_ratio_mixed_over_fp64": drift_ratio,
         "largest_cluster": res["mixed"]["largest"],
         "rebuilt_atom_fraction_per_step": res["mixed"]["rebuild_frac"],
+        "tuning": {"mixed": res["mixed"]["tuning"], "fp64": res["fp64"]["tuning"]},
         "trajectories_reaching_fpt": res["mixed"]["nfpt"],
         "kernel_ms_per_step": kt,
         "step_roofline_frac": step_bytes * K / res["mixed"]["wall"] / HBM_PEAK,

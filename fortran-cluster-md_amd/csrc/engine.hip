@@ -326,6 +326,9 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     s->g.nblk = (int)(abase / BLK);
     s->g.ncblk = (int)(cbase / SCAN_CHUNK);
     s->g.ntraj = ntraj;
+    s->g.force_u = 4;
+    s->g.nbr_cu = 2;
+    s->g.wl_grid = 2048;
 
     DevPtrs &d = s->d;
     const long long N = abase;
@@ -759,6 +762,104 @@ int fcmd_last_run_ms(fcmd_system *s, double *total_ms)
     if (!s || !total_ms) return FCMD_ERR_ARG;
     *total_ms = s->last_ms;
     return FCMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tuning.  The variants differ only in how many loads are in flight and in
+// launch widths: every one sums the same terms in the same order.
+
+namespace {
+const int FORCE_U[] = {2, 4, 8};
+const int NBR_CU[] = {2, 4};
+const int WL_GRIDS[] = {1024, 2048, 4096};
+
+bool one_of(int v, const int *set, int n)
+{
+    for (int k = 0; k < n; ++k)
+        if (set[k] == v) return true;
+    return false;
+}
+
+void drop_graph(fcmd_system *s)
+{
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+}
+}  // namespace
+
+int fcmd_set_tuning(fcmd_system *s, const int *tuning)
+{
+    if (!s || !tuning || !one_of(tuning[0], FORCE_U, 3) || !one_of(tuning[1], NBR_CU, 2) ||
+        !one_of(tuning[2], WL_GRIDS, 3))
+        return FCMD_ERR_ARG;
+    s->g.force_u = tuning[0];
+    s->g.nbr_cu = tuning[1];
+    s->g.wl_grid = tuning[2];
+    drop_graph(s);
+    return FCMD_OK;
+}
+
+int fcmd_get_tuning(fcmd_system *s, int *tuning)
+{
+    if (!s || !tuning) return FCMD_ERR_ARG;
+    tuning[0] = s->g.force_u;
+    tuning[1] = s->g.nbr_cu;
+    tuning[2] = s->g.wl_grid;
+    return FCMD_OK;
+}
+
+int fcmd_autotune(fcmd_system *s, int rounds, int *tuning)
+{
+    if (!s || rounds <= 0) return FCMD_ERR_ARG;
+    finalize(s);
+    const int nf = 3, nr = 2 * 3;
+    // one step per candidate and round, the force and rebuild candidates
+    // cycling independently so that each sees a like mix of rebuilds
+    const int nsteps = rounds * nr;
+    std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+    for (auto &e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    const LaunchDims keep = s->g;
+    for (int k = 0; k < nsteps; ++k) {
+        hipEvent_t *e = ev.data() + (size_t)k * 4;
+        LaunchDims g = keep;
+        g.force_u = FORCE_U[k % nf];
+        g.nbr_cu = NBR_CU[(k % nr) / 3];
+        g.wl_grid = WL_GRIDS[(k % nr) % 3];
+        launch_drift(s->d, g, s->mixed, s->stream);
+        HIPCHK(hipEventRecord(e[0], s->stream));
+        launch_rebuild(s->d, g, s->mixed, s->stream);
+        HIPCHK(hipEventRecord(e[1], s->stream));
+        launch_force_atoms(s->d, g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        HIPCHK(hipEventRecord(e[2], s->stream));
+        launch_traj_post(s->d, g, FORCE_STEP, s->stream);
+        HIPCHK(hipEventRecord(e[3], s->stream));
+    }
+    HIPCHK(hipGetLastError());
+    finalize(s);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double tf[3] = {0, 0, 0}, tr[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < nsteps; ++k) {
+        hipEvent_t *e = ev.data() + (size_t)k * 4;
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+        HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+        tr[k % nr] += a;
+        tf[k % nf] += b;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    int bf = 0, br = 0;
+    for (int c = 1; c < nf; ++c)
+        if (tf[c] < tf[bf]) bf = c;
+    for (int c = 1; c < nr; ++c)
+        if (tr[c] < tr[br]) br = c;
+    s->g.force_u = FORCE_U[bf];
+    s->g.nbr_cu = NBR_CU[br / 3];
+    s->g.wl_grid = WL_GRIDS[br % 3];
+    drop_graph(s);
+    if (tuning) fcmd_get_tuning(s, tuning);
+    return sync_err(s);
 }
 
 int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)

@@ -14,6 +14,10 @@ struct LaunchDims {
     int ncblk;   // cell chunks
     int nhblk;   // histogram chunks
     int ntraj;
+    // tunables (none changes a result bit; fcmd_autotune picks them):
+    int force_u;  // neighbours per load batch in the force kernel: 2, 4 or 8
+    int nbr_cu;   // candidates per load batch in the list search: 2 or 4
+    int wl_grid;  // workgroups of each rebuild kernel: 1024, 2048 or 4096
 };
 
 // launch_drift = launch_drift_atoms (k_drift) + launch_traj_pre (k_traj_pre);

@@ -23,7 +23,6 @@
 
 namespace fcmd {
 
-constexpr int WL_GRID = 2048;  // blocks of the work-list kernels (8 per CU)
 
 __device__ __forceinline__ bool rebuilding(const TrajScalars &ts) { return ts.rebuild && !ts.err; }
 
@@ -333,9 +332,9 @@ __global__ __launch_bounds__(BLK) void k_copyback(DevPtrs d)
 // are one contiguous slot range, or two where the row wraps round the box.
 // The bounds of a z-plane's rows are loaded together before its first
 // candidate; each row is then walked as one sequence (its second range
-// following the first), CU candidates per iteration with their loads
-// issued together.  The visiting order is the same as cell by cell.
-template <typename Sink>
+// following the first), CU candidates (a tunable) per iteration with their
+// loads issued together.  The visiting order is the same as cell by cell.
+template <int CU, typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                           int i, int &half, Sink sink)
 {
@@ -363,7 +362,6 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     }
     const int *cstart = d.cell_start + tp.cbase;
     const int *ccnt = d.cell_cnt + tp.cbase;
-    constexpr int CU = 2;  // candidates whose loads are issued together
     int cnt = 0;
 #pragma unroll 1
     for (int oz = lz; oz <= hz; ++oz) {
@@ -444,6 +442,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
 // neighbours stops the trajectory with LISTFULL (the reference stops when its
 // list array is nearly full, neighbor-pbc.f:89-93); its count is clamped so
 // that no later kernel reads past its slots.
+template <int CU>
 __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
 {
     __shared__ int ldsi[4];
@@ -458,7 +457,7 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         int cnt = 0, half = 0;
         if (rel < tp.natom) {
             int *out = d.list + list_slot(tp, rel);
-            cnt = candidates(d, tp, ts, i, half, [&](int j, int k) {
+            cnt = candidates<CU>(d, tp, ts, i, half, [&](int j, int k) {
                 if (k < width) out[(size_t)k * TILE] = j;
             });
         }
@@ -478,8 +477,9 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
 {
     const dim3 blk(BLK);
-    const dim3 ga(xcd_grid(std::min(g.nblk, WL_GRID))), gc(xcd_grid(std::min(g.ncblk, WL_GRID)));
-    const dim3 gj(xcd_grid(std::min(g.ntraj, WL_GRID)));
+    const int wg = g.wl_grid;  // workgroups per rebuild kernel (a tunable)
+    const dim3 ga(xcd_grid(std::min(g.nblk, wg))), gc(xcd_grid(std::min(g.ncblk, wg)));
+    const dim3 gj(xcd_grid(std::min(g.ntraj, wg)));
     hipLaunchKernelGGL(k_wl_fill, dim3(g.ntraj), blk, 0, s, d, g.ntraj);
     hipLaunchKernelGGL(k_zero_cells, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_bin, ga, blk, 0, s, d);
@@ -495,7 +495,10 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
         hipLaunchKernelGGL(k_permute<double>, ga, blk, 0, s, d);
         hipLaunchKernelGGL(k_copyback<double>, ga, blk, 0, s, d);
     }
-    hipLaunchKernelGGL(k_nbr, ga, blk, 0, s, d);
+    if (g.nbr_cu == 4)
+        hipLaunchKernelGGL(k_nbr<4>, ga, blk, 0, s, d);
+    else
+        hipLaunchKernelGGL(k_nbr<2>, ga, blk, 0, s, d);
 }
 
 }  // namespace fcmd

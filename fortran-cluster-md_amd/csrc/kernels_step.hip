@@ -340,12 +340,11 @@ __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
 //
 // FP64: the reference's arithmetic, minimum image in its form, so every pair
 // term and cutoff decision is the reference's, bit for bit.
-template <int FORM>
+template <int FORM, int U>
 __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                               int i, int rel, int ti, double &fx, double &fy, double &fz,
                                               double &e)
 {
-    constexpr int U = 4;
     const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
     const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
@@ -399,26 +398,30 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
 
 // one coordinate of a mixed-path separation: cell difference wrapped to a
 // neighbour (every listed pair was in neighbouring cells at the list build),
-// then, for a box of fewer than 3 cells, the remaining image by rounding
+// then, when SMALL (some side of the box has fewer than 3 cells), the
+// remaining image by rounding on the sides that need it
+template <bool SMALL>
 __device__ __forceinline__ float sep_mixed(int ci, int cj, int nc, float side, float off, float box,
                                            float ibox)
 {
     int dc = ci - cj;
     dc += dc > 1 ? -nc : (dc < -1 ? nc : 0);
     float s = (float)dc * side + off;
-    if (nc < 3) s -= box * rintf(s * ibox);
+    if (SMALL && nc < 3) s -= box * rintf(s * ibox);
     return s;
 }
 
 // Mixed: separations from the FP32 cell-offset records (make_posf: one 16-byte
-// gather per neighbour, no FP64), pair distance and LJ terms in FP32, forces
-// summed in FP32 with Kahan compensation, energies summed in FP64.
-template <int FORM>
+// gather per neighbour, no FP64), pair distance and LJ terms in FP32 (1/r^2 by
+// the hardware reciprocal, 1 ulp), forces summed in FP32 with Kahan
+// compensation, energies summed in FP64.  SMALL: a separate loop for boxes
+// with fewer than 3 cells on some side, so the usual case carries no rounding
+// image.
+template <int FORM, bool SMALL, int U>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, int rel, int ti, double &fx, double &fy, double &fz,
                                                double &e)
 {
-    constexpr int U = 4;
     const float4 pi = d.posf[i];
     int cxi, cyi, czi;
     unpack_cell(pi.w, cxi, cyi, czi);
@@ -445,12 +448,12 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
             if (s0 + u >= nn) break;
             int cxj, cyj, czj;
             unpack_cell(pj[u].w, cxj, cyj, czj);
-            const float dxf = sep_mixed(cxi, cxj, ncx, sx, pi.x - pj[u].x, bx, ibx);
-            const float dyf = sep_mixed(cyi, cyj, ncy, sy, pi.y - pj[u].y, by, iby);
-            const float dzf = sep_mixed(czi, czj, ncz, sz, pi.z - pj[u].z, bz, ibz);
+            const float dxf = sep_mixed<SMALL>(cxi, cxj, ncx, sx, pi.x - pj[u].x, bx, ibx);
+            const float dyf = sep_mixed<SMALL>(cyi, cyj, ncy, sy, pi.y - pj[u].y, by, iby);
+            const float dzf = sep_mixed<SMALL>(czi, czj, ncz, sz, pi.z - pj[u].z, bz, ibz);
             const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
             if (r2 <= rc2f) {
-                const float ri2 = 1.0f / r2;
+                const float ri2 = __builtin_amdgcn_rcpf(r2);
                 const float r6 = ri2 * ri2 * ri2;
                 float fij;
                 if (FORM == ACC_1) {
@@ -472,21 +475,25 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     fz = (double)kz.s;
 }
 
-template <typename AccT, int FORM>
+template <typename AccT, int FORM, int U>
 __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                          int i, int rel, int ti, double &fx, double &fy, double &fz,
                                          double &e)
 {
-    if constexpr (sizeof(AccT) == 4)
-        pair_sum_mixed<FORM>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
-    else
-        pair_sum_fp64<FORM>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+    if constexpr (sizeof(AccT) == 4) {
+        if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
+            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+        else
+            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+    } else
+        pair_sum_fp64<FORM, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
 }
 
 // LJ forces over the full neighbour list plus the integrator's fused
 // epilogue.  MULTI = false: every trajectory of the launch is single-species
-// (ACC_1); MULTI = true: each trajectory uses its own form.
-template <typename AccT, bool MULTI>
+// (ACC_1); MULTI = true: each trajectory uses its own form.  U: neighbours
+// per load batch (a tunable; the pairs are summed in list order whatever U).
+template <typename AccT, bool MULTI, int U>
 __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
@@ -506,9 +513,9 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
         const int ti = d.typ[i];
         double fx, fy, fz;
         if (form1)
-            pair_sum<AccT, ACC_1>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+            pair_sum<AccT, ACC_1, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
         else
-            pair_sum<AccT, ACC_M>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+            pair_sum<AccT, ACC_M, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
         // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:408-414
         double ax, ay, az;
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
@@ -641,20 +648,31 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
     launch_traj_pre(d, g, s);
 }
 
+template <typename AccT, bool MULTI>
+static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+{
+    const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
+    if (g.force_u == 2)
+        hipLaunchKernelGGL((k_force<AccT, MULTI, 2>), grid, blk, 0, s, d, mode);
+    else if (g.force_u == 8)
+        hipLaunchKernelGGL((k_force<AccT, MULTI, 8>), grid, blk, 0, s, d, mode);
+    else
+        hipLaunchKernelGGL((k_force<AccT, MULTI, 4>), grid, blk, 0, s, d, mode);
+}
+
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
                         hipStream_t s)
 {
-    const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
     if (mixed) {
         if (single_species)
-            hipLaunchKernelGGL((k_force<float, false>), grid, blk, 0, s, d, mode);
+            launch_force_u<float, false>(d, g, mode, s);
         else
-            hipLaunchKernelGGL((k_force<float, true>), grid, blk, 0, s, d, mode);
+            launch_force_u<float, true>(d, g, mode, s);
     } else {
         if (single_species)
-            hipLaunchKernelGGL((k_force<double, false>), grid, blk, 0, s, d, mode);
+            launch_force_u<double, false>(d, g, mode, s);
         else
-            hipLaunchKernelGGL((k_force<double, true>), grid, blk, 0, s, d, mode);
+            launch_force_u<double, true>(d, g, mode, s);
     }
 }
 

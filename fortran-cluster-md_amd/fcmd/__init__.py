@@ -86,7 +86,8 @@ EXPORTS = ("fcmd_device_count", "fcmd_error_text", "fcmd_create", "fcmd_destroy"
            "fcmd_set_params", "fcmd_upload", "fcmd_download", "fcmd_set_scalars",
            "fcmd_get_scalars", "fcmd_neighbor", "fcmd_accatom", "fcmd_integrator",
            "fcmd_stoddard", "fcmd_halflist", "fcmd_run", "fcmd_init_random",
-           "fcmd_last_run_ms", "fcmd_time_kernels", "fcmd_list_entries")
+           "fcmd_last_run_ms", "fcmd_time_kernels", "fcmd_list_entries", "fcmd_set_tuning",
+           "fcmd_get_tuning", "fcmd_autotune")
 
 
 def lib():
@@ -298,6 +299,23 @@ class System:
         out = np.zeros(len(TIMED_KERNELS))
         check(lib().fcmd_time_kernels(self.h, int(nsteps), _p(out, _PD)))
         return dict(zip(TIMED_KERNELS, out.tolist()))
+
+    def tuning(self) -> dict:
+        """Current launch tunables (force_u, nbr_cu, wl_grid)."""
+        t = np.zeros(3, dtype=np.int32)
+        check(lib().fcmd_get_tuning(self.h, _p(t, _PI)))
+        return dict(zip(("force_u", "nbr_cu", "wl_grid"), t.tolist()))
+
+    def set_tuning(self, force_u: int, nbr_cu: int, wl_grid: int):
+        t = np.array([force_u, nbr_cu, wl_grid], dtype=np.int32)
+        check(lib().fcmd_set_tuning(self.h, _p(t, _PI)))
+
+    def autotune(self, rounds: int = 5) -> dict:
+        """Time every tunable candidate over rounds x 6 real steps and keep
+        the fastest (the steps advance the state like run())."""
+        t = np.zeros(3, dtype=np.int32)
+        check(lib().fcmd_autotune(self.h, int(rounds), _p(t, _PI)))
+        return dict(zip(("force_u", "nbr_cu", "wl_grid"), t.tolist()))
 
     def list_entries(self) -> int:
         v = C.c_longlong()

@@ -172,6 +172,21 @@ int fcmd_last_run_ms(fcmd_system *s, double *total_ms);
 #define FCMD_NTIMED 6
 int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms);
 
+/* Launch tunables: tuning[3] = {neighbours per load batch in the force
+ * kernel (2, 4, 8), candidates per load batch in the list search (2, 4),
+ * workgroups per rebuild kernel (1024, 2048, 4096)}.  No tunable changes any
+ * result bit.  Defaults {4, 2, 2048}.  No counterpart in the reference (its
+ * loops have no launch shape). */
+int fcmd_set_tuning(fcmd_system *s, const int *tuning);
+int fcmd_get_tuning(fcmd_system *s, int *tuning);
+
+/* Autotuner: runs rounds x 6 integrator steps of every trajectory (real
+ * steps: the state advances as in fcmd_run), the candidates taking turns
+ * step by step, times the force kernel and the rebuild sequence of each on
+ * the engine stream, and keeps the fastest; tuning (may be NULL) receives
+ * the choice.  No counterpart in the reference. */
+int fcmd_autotune(fcmd_system *s, int rounds, int *tuning);
+
 /* Full-list entries (2 x pairs) summed over all trajectories' current lists. */
 int fcmd_list_entries(fcmd_system *s, long long *entries);
 

@@ -240,3 +240,22 @@ def test_mixed_precision_force_error():
     den = sum((o.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
     assert np.sqrt(num / den) < 1e-5
     assert abs(s.scalars().epot - o.st.epot) < 1e-5 * abs(o.st.epot)
+
+
+@pytest.mark.parametrize("n,rho", [(8, 0.05), (12, 0.05), (100, 0.05)])
+def test_mixed_precision_small_boxes(n, rho):
+    """Boxes of 1, 2 and 4 cells a side (list radius 2.8 sigma): the mixed
+    path's cell-offset separations need the rounding image below 3 cells."""
+    case = cases.make_case(n=n, rho=rho, seed=43, dense_frac=0.0)
+    o = po.Oracle(case, "mav")
+    o.neighbor(0)
+    o.accatom()
+    s = gpu_system(case, "mav", precision=fcmd.MIXED)
+    s.neighbor(0)
+    s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
+    s.accatom()
+    got = s.download()
+    num = sum(((got[k] - o.arr[k]) ** 2).sum() for k in ("ax", "ay", "az"))
+    den = sum((o.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
+    assert np.sqrt(num / den) < 1e-5
+    assert abs(s.scalars().epot - o.st.epot) < 1e-5 * abs(o.st.epot)

@@ -1,0 +1,53 @@
+"""The launch tunables (force-kernel load batch, list-search load batch,
+rebuild grid) and the autotuner (needs a GPU).  No tunable may change a
+single bit of the trajectory: every variant sums the same terms in the same
+order."""
+import numpy as np
+import pytest
+
+import fcmd
+from fcmd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [(4, 2, 2048), (2, 4, 1024), (8, 2, 4096), (8, 4, 2048)]
+
+
+def _small_ensemble():
+    params, states, _ = W.ensemble512(ntraj=12, nmin=300, nmax=6000)
+    return params, states
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+def test_every_tuning_gives_the_same_bits(precision):
+    params, states = _small_ensemble()
+    outs = []
+    for cfg in CONFIGS:
+        with fcmd.System(params, precision) as s:
+            s.set_tuning(*cfg)
+            assert tuple(s.tuning().values()) == cfg
+            W.setup(s, params, states)
+            s.run(60, use_graph=True)
+            outs.append([(s.download(t), s.scalars(t).epot, s.scalars(t).psi, s.scalars(t).nrebuild)
+                         for t in range(len(params))])
+    for other in outs[1:]:
+        for (a, ea, pa, ra), (b, eb, pb, rb) in zip(outs[0], other):
+            for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+                np.testing.assert_array_equal(a[k], b[k])
+            assert (ea, pa, ra) == (eb, pb, rb)
+
+
+def test_autotune_picks_a_candidate_and_the_run_goes_on():
+    params, states = _small_ensemble()
+    with fcmd.System(params, fcmd.MIXED) as s:
+        W.setup(s, params, states)
+        before = [s.scalars(t).step for t in range(len(params))]
+        got = s.autotune(rounds=2)
+        assert got["force_u"] in (2, 4, 8) and got["nbr_cu"] in (2, 4)
+        assert got["wl_grid"] in (1024, 2048, 4096)
+        assert s.tuning() == got
+        # the tuning steps are ordinary steps of every trajectory
+        assert [s.scalars(t).step - b for t, b in enumerate(before)] == [12] * len(params)
+        s.run(20, use_graph=True)
+        with pytest.raises(fcmd.FcmdError):
+            s.set_tuning(3, 2, 2048)
