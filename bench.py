@@ -229,6 +229,12 @@ def main():
             kt = s.time_kernels(20)
             res[name]["kernels"] = kt
             res[name]["nlist_k"] = s.list_entries()
+            # the drop-in hands host buffers over the boundary: one download
+            # of every trajectory's x, v, a (what an output step needs)
+            t0 = time.perf_counter()
+            for t in range(len(params)):
+                s.download(t)
+            res[name]["download_s"] = time.perf_counter() - t0
         s.close()
 
     plain = None
@@ -310,6 +316,8 @@ def main():
         "energy_drift_ratio_mixed_over_fp64": drift_ratio,
         "largest_cluster": res["mixed"]["largest"],
         "rebuilt_atom_fraction_per_step": res["mixed"]["rebuild_frac"],
+        "state_download_ms": res["mixed"]["download_s"] * 1e3,
+        "pcie_inclusive_output_every_100": natoms * K / (res["mixed"]["wall"] + K / 100 * res["mixed"]["download_s"]),
         "tuning": {"mixed": res["mixed"]["tuning"], "fp64": res["fp64"]["tuning"]},
         "trajectories_reaching_fpt": res["mixed"]["nfpt"],
         "kernel_ms_per_step": kt,
