@@ -110,6 +110,7 @@ struct fcmd_system {
     int *dstep = nullptr;       // device int for step round trips
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int count_nmax = 0, count_warmup = 0;  // fcmd_size_counts (0: off)
     double last_ms = 0.0;
 
     template <typename T>
@@ -342,7 +343,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.x2, N) A(d.y2, N) A(d.z2, N) A(d.vx2, N) A(d.vy2, N) A(d.vz2, N);
     if (s->mixed) {
         A(d.axf, N) A(d.ayf, N) A(d.azf, N) A(d.axf2, N) A(d.ayf2, N) A(d.azf2, N);
-        A(d.posf, N);
+        A(d.posf, N) A(d.cellid, N);
     } else {
         A(d.ax, N) A(d.ay, N) A(d.az, N) A(d.ax2, N) A(d.ay2, N) A(d.az2, N);
     }
@@ -361,6 +362,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.o_cek, N);
     A(d.o_csi, hbase) A(d.o_csf, hbase) A(d.o_csif, hbase) A(d.o_cgsf, hbase) A(d.o_cgsfa, hbase);
     A(d.o_ceks, hbase) A(d.o_cekf, hbase) A(d.o_ceksf, hbase);
+    A(d.o_hacc, hbase) A(d.hacc_stop, ntraj);
     long long maxn = 1;
     for (auto &p : s->hp) maxn = std::max<long long>(maxn, p.natom + 1);
     A(s->stage, maxn) A(s->stage_i, maxn) A(s->dstep, 4);
@@ -732,6 +734,8 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
             launch_cluster(s->d, s->g, s->any_fre, s->stream);
             hipLaunchKernelGGL(k_record, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->d.ts,
                                s->ntraj, dseries + (size_t)snap * s->ntraj, dfpt, fpt_size, step0[0] + done);
+            if (s->count_nmax > 0)
+                launch_size_counts(s->d, s->g, s->count_nmax, s->count_warmup, step0[0] + done, s->stream);
             ++snap;
         }
         const int todo = std::min(nsteps - done, nsnap > 0 ? cluster_every - done % cluster_every : nsteps - done);
@@ -755,6 +759,29 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
         (void)hipFree(dfpt);
     }
     return sync_err(s);
+}
+
+int fcmd_size_counts(fcmd_system *s, int nmax, int warmup)
+{
+    if (!s) return FCMD_ERR_ARG;
+    finalize(s);
+    HIPCHK(hipMemsetAsync(s->d.o_hacc, 0, sizeof(int) * s->nhist_total, s->stream));
+    HIPCHK(hipMemsetAsync(s->d.hacc_stop, 0, sizeof(int) * s->ntraj, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->count_nmax = nmax > 0 ? nmax : 0;
+    s->count_warmup = warmup;
+    return FCMD_OK;
+}
+
+int fcmd_get_size_counts(fcmd_system *s, int traj, int *counts, int *stopped)
+{
+    if (check_traj(s, traj) || !counts) return FCMD_ERR_ARG;
+    const TrajParams &t = s->tp[traj];
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(counts, s->d.o_hacc + t.hbase, sizeof(int) * (t.natom + 1), hipMemcpyDeviceToHost));
+    counts[0] = 0;
+    if (stopped) HIPCHK(hipMemcpy(stopped, s->d.hacc_stop + traj, sizeof(int), hipMemcpyDeviceToHost));
+    return FCMD_OK;
 }
 
 int fcmd_last_run_ms(fcmd_system *s, double *total_ms)

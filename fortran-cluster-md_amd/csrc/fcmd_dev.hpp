@@ -140,6 +140,7 @@ struct DevPtrs {
     // from the centre of its cell at the last list build, and that cell packed
     // in w (make_posf)
     float4 *posf;
+    int *cellid;                     // mixed path: the packed cell of posf.w, alone (the drift reads it)
     // clusters: per slot
     ClusterScalars *cls;
     int *edeg, *edeg1, *edges, *parent, *parentf, *liq;
@@ -152,6 +153,8 @@ struct DevPtrs {
     // clusters: per size (hbase + size)
     int *o_csi, *o_csf, *o_csif, *o_cgsf, *o_cgsfa;
     double *o_ceks, *o_cekf, *o_ceksf;
+    int *o_hacc;                     // cluster-size counts over analyses (hbase + size)
+    int *hacc_stop;                  // per trajectory: counting has stopped
 };
 
 }  // namespace fcmd

@@ -36,5 +36,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
 // after positions of trajectory t are replaced from the host (mixed path)
 void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s);
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s);
+// after launch_cluster: sizer-style cluster-size counts of this analysis
+void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int warmup, int step, hipStream_t s);
 
 }  // namespace fcmd

@@ -12,6 +12,8 @@
 // atom number, which is exactly the reference's label cl(i).  The ten
 // Wolde-Frenkel label is the 1-based rank, among liquid-like atoms in atom
 // order, of the component's lowest member (stoddard-pbc-n-m-fre.f:198-225).
+#include <climits>
+
 #include "fcmd_common.hpp"
 #include "fcmd_kernels.hpp"
 
@@ -323,6 +325,38 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
         d.o_clf[o] = lab;
         d.o_ccf[o] = 0;
     }
+}
+
+// Cluster-size probability counts, helpers/sizer.f:59-103 (Stillinger
+// histogram, SWITCH = 1): one block per trajectory adds this analysis's
+// csi(g) to its counts for g = 1 .. the first size >= nmax present
+// (inclusive: sizer counts that row, then sets its stop flag), and stops the
+// trajectory there.  Snapshots before step `warmup` add nothing but can stop
+// the counting, as in sizer (its flag is set whatever the step).
+__global__ __launch_bounds__(BLK) void k_size_counts(DevPtrs d, int nmax, int warmup, int step)
+{
+    const int t = blockIdx.x;
+    if (d.hacc_stop[t] || !d.cls[t].want || d.ts[t].err) return;
+    const TrajParams &tp = d.tp[t];
+    const int *csi = d.o_csi + tp.hbase;
+    __shared__ int first;
+    if (threadIdx.x == 0) first = INT_MAX;
+    __syncthreads();
+    for (int g = max(nmax, 1) + threadIdx.x; g <= tp.natom; g += BLK)
+        if (csi[g] > 0) {
+            atomicMin(&first, g);
+            break;
+        }
+    __syncthreads();
+    const int upto = first == INT_MAX ? tp.natom : first;
+    if (step >= warmup)
+        for (int g = 1 + threadIdx.x; g <= upto; g += BLK) d.o_hacc[tp.hbase + g] += csi[g];
+    if (threadIdx.x == 0 && first != INT_MAX) d.hacc_stop[t] = 1;
+}
+
+void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int warmup, int step, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_size_counts, dim3(g.ntraj), dim3(BLK), 0, s, d, nmax, warmup, step);
 }
 
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s)

@@ -8,9 +8,10 @@
 // which no trajectory rebuilds costs a few empty launches, and the sequence
 // can be captured in a graph because no host decision is involved.
 //
-//   wl_fill -> zero_cells -> bin (wrap + cell + slot) -> scan(cell counts)
+//   wl_fill -> zero_cells -> bin (wrap + cell + slot) -> scan (cell counts)
 //   -> scatter -> cellsort (deterministic order inside a cell) -> permute
-//   -> copyback -> nbr (search + list, one pass)
+//   (into the alternate buffers) -> nbr (search + list in one pass, then the
+//   copy back)
 //
 // The pair test is the reference's: on wrapped positions,
 //   d = x(i) - x(j);  d = d + L - L*int(1.5 + d/L);  r2 <= atrskin^2
@@ -108,8 +109,9 @@ __global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
 
 // ---------------------------------------------------------------------------
 // cell_start = exclusive scan of cell_cnt within each trajectory: a scan of
-// each SCAN_CHUNK-cell chunk, a scan of each trajectory's chunk totals, and
-// the add of the chunk offsets
+// each SCAN_CHUNK-cell chunk (k_scan_chunk), a scan of each trajectory's chunk
+// totals (k_scan_totals); the chunk offset is added where a start is first
+// used (k_scatter reads start + offset, k_cellsort writes the final start)
 __global__ __launch_bounds__(BLK) void k_scan_chunk(DevPtrs d)
 {
     __shared__ long long lds[4];
@@ -164,16 +166,6 @@ __global__ __launch_bounds__(BLK) void k_scan_totals(DevPtrs d)
     }
 }
 
-__global__ __launch_bounds__(BLK) void k_scan_add(DevPtrs d)
-{
-    FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
-    {
-        const int add = (int)d.scan_tmp_ll[c];
-        const int base = c * SCAN_CHUNK + threadIdx.x * 4;
-        for (int k = 0; k < 4; ++k) d.cell_start[base + k] += add;
-    }
-}
-
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLK) void k_zero_cells(DevPtrs d)
 {
@@ -223,24 +215,29 @@ __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
         const TrajParams &tp = d.tp[d.blk_traj[b]];
         const int i = b * BLK + threadIdx.x;
         if (i - tp.abase < tp.natom) {
-            const int c = d.atom_cell[i];
-            d.sorted[tp.abase + d.cell_start[tp.cbase + c] + d.atom_slot[i]] = i;
+            const int c = tp.cbase + d.atom_cell[i];
+            const int start = d.cell_start[c] + (int)d.scan_tmp_ll[c / SCAN_CHUNK];
+            d.sorted[tp.abase + start + d.atom_slot[i]] = i;
         }
     }
 }
 
 // insertion sort of each cell's atoms by reference atom number, so the
-// order (and every later floating-point sum) does not depend on atomics
+// order (and every later floating-point sum) does not depend on atomics; also
+// writes each cell's final start
 __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
 {
     FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
         const TrajParams &tp = d.tp[d.cblk_traj[c]];
+        const int add = (int)d.scan_tmp_ll[c];
         for (int k = 0; k < 4; ++k) {
             const int cell = c * SCAN_CHUNK + threadIdx.x * 4 + k;
+            const int start = d.cell_start[cell] + add;
+            d.cell_start[cell] = start;
             const int n = d.cell_cnt[cell];
             if (n < 2) continue;
-            int *a = d.sorted + tp.abase + d.cell_start[cell];
+            int *a = d.sorted + tp.abase + start;
             for (int p = 1; p < n; ++p) {
                 const int v = a[p];
                 const int key = d.oid[v];
@@ -271,7 +268,9 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
             const TrajScalars &ts = d.ts[t];
             const int c = d.atom_cell[o];
             const int cx = c % ts.ncx, cy = (c / ts.ncx) % ts.ncy, cz = c / (ts.ncx * ts.ncy);
-            d.posf[i] = make_posf(d.x[o], d.y[o], d.z[o], cx, cy, cz, ts);
+            const float4 r = make_posf(d.x[o], d.y[o], d.z[o], cx, cy, cz, ts);
+            d.posf[i] = r;
+            d.cellid[i] = __float_as_int(r.w);
         }
         d.x2[i] = d.x[o];
         d.y2[i] = d.y[o];
@@ -293,40 +292,12 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
     }
 }
 
-template <typename AccT>
-__global__ __launch_bounds__(BLK) void k_copyback(DevPtrs d)
-{
-    FOR_ITEMS(d.wl_blk, WL_BLK, b)
-    {
-        const TrajParams &tp = d.tp[d.blk_traj[b]];
-        const int i = b * BLK + threadIdx.x;
-        d.x[i] = d.x2[i];
-        d.y[i] = d.y2[i];
-        d.z[i] = d.z2[i];
-        d.vx[i] = d.vx2[i];
-        d.vy[i] = d.vy2[i];
-        d.vz[i] = d.vz2[i];
-        if (sizeof(AccT) == 8) {
-            d.ax[i] = d.ax2[i];
-            d.ay[i] = d.ay2[i];
-            d.az[i] = d.az2[i];
-        } else {
-            d.axf[i] = d.axf2[i];
-            d.ayf[i] = d.ayf2[i];
-            d.azf[i] = d.azf2[i];
-        }
-        d.typ[i] = d.typ2[i];
-        const int o = d.oid2[i];
-        d.oid[i] = o;
-        if (i - tp.abase < tp.natom) d.inv[tp.abase + o - 1] = i;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// The candidate search: every atom j != i of the 27 (or fewer, without
-// duplicates) cells around i's cell whose reference r2 is within the list
-// radius, in a fixed order (cells z-major, then y, then x; atoms of one cell
-// by reference number); sink(j, cnt) is called for each.
+// The candidate search, on the permuted arrays (x2, oid2: cell order): every
+// atom j != i of the 27 (or fewer, without duplicates) cells around i's cell
+// whose reference r2 is within the list radius, in a fixed order (cells
+// z-major, then y, then x; atoms of one cell by reference number); sink(j,
+// cnt) is called for each.
 //
 // Atoms are sorted by cell, so the x-neighbouring cells of one (y, z) row
 // are one contiguous slot range, or two where the row wraps round the box.
@@ -341,8 +312,8 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
     const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
     const double rs2 = ts.rs2;
-    const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
-    const int oi = d.oid[i];
+    const double xi = d.x2[i], yi = d.y2[i], zi = d.z2[i];
+    const int oi = d.oid2[i];
     const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
     const int cx = cell_coord(xi, ts.cside[0], ncx);
     const int cy = cell_coord(yi, ts.cside[1], ncy);
@@ -401,10 +372,10 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                 for (int u = 0; u < CU; ++u) {
                     const int k = min(k0 + u, n - 1);
                     jj[u] = k < na ? base_a + k : base_b + k;
-                    oj[u] = d.oid[jj[u]];
-                    xj[u] = d.x[jj[u]];
-                    yj[u] = d.y[jj[u]];
-                    zj[u] = d.z[jj[u]];
+                    oj[u] = d.oid2[jj[u]];
+                    xj[u] = d.x2[jj[u]];
+                    yj[u] = d.y2[jj[u]];
+                    zj[u] = d.z2[jj[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < CU; ++u) {
@@ -438,7 +409,8 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
 }
 
 // The list, in one pass: search, write each atom's neighbours to its lwidth
-// slots, count the pairs and the widest atom.  An atom with more than lwidth
+// slots, count the pairs and the widest atom, then copy the block's permuted
+// arrays back.  An atom with more than lwidth
 // neighbours stops the trajectory with LISTFULL (the reference stops when its
 // list array is nearly full, neighbor-pbc.f:89-93); its count is clamped so
 // that no later kernel reads past its slots.
@@ -470,6 +442,27 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         if ((threadIdx.x & (TILE - 1)) == 0 && w > 0) atomicMax(&ts.nlist, w);
         const int h = block_sum_i(half, ldsi);
         if (threadIdx.x == 0 && h) atomicAdd(&ts.npairs, h);
+        // copy back: every slot of the block takes its permuted arrays (no
+        // thread of this kernel reads the primary ones)
+        d.x[i] = d.x2[i];
+        d.y[i] = d.y2[i];
+        d.z[i] = d.z2[i];
+        d.vx[i] = d.vx2[i];
+        d.vy[i] = d.vy2[i];
+        d.vz[i] = d.vz2[i];
+        if (d.axf) {
+            d.axf[i] = d.axf2[i];
+            d.ayf[i] = d.ayf2[i];
+            d.azf[i] = d.azf2[i];
+        } else {
+            d.ax[i] = d.ax2[i];
+            d.ay[i] = d.ay2[i];
+            d.az[i] = d.az2[i];
+        }
+        d.typ[i] = d.typ2[i];
+        const int o = d.oid2[i];
+        d.oid[i] = o;
+        if (rel < tp.natom) d.inv[tp.abase + o - 1] = i;
     }
 }
 
@@ -485,16 +478,12 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     hipLaunchKernelGGL(k_bin, ga, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scan_add, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, ga, blk, 0, s, d);
     hipLaunchKernelGGL(k_cellsort, gc, blk, 0, s, d);
-    if (mixed) {
+    if (mixed)
         hipLaunchKernelGGL(k_permute<float>, ga, blk, 0, s, d);
-        hipLaunchKernelGGL(k_copyback<float>, ga, blk, 0, s, d);
-    } else {
+    else
         hipLaunchKernelGGL(k_permute<double>, ga, blk, 0, s, d);
-        hipLaunchKernelGGL(k_copyback<double>, ga, blk, 0, s, d);
-    }
     if (g.nbr_cu == 4)
         hipLaunchKernelGGL(k_nbr<4>, ga, blk, 0, s, d);
     else

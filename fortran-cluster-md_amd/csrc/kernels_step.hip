@@ -183,7 +183,7 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
         double dx, dy, dz;
         if (tp.integrator == INT_NH) {
             const double dt2 = dt * dt / 2, dth = dt / 2;
-            const int ty = d.typ[i];
+            const int ty = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
             const bool cpl = nh_coupled(tp, ty);
             const double psi = cpl ? ts.psi : 0.0;
             if (ts.pending) {
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
         d.z[i] = zn;
         if (sizeof(AccT) == 4) {
             int cx, cy, cz;
-            unpack_cell(d.posf[i].w, cx, cy, cz);
+            unpack_cell(__int_as_float(d.cellid[i]), cx, cy, cz);
             d.posf[i] = make_posf(xn, yn, zn, cx, cy, cz, ts);
         }
         d.vx[i] = vx;
@@ -340,10 +340,30 @@ __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
 //
 // FP64: the reference's arithmetic, minimum image in its form, so every pair
 // term and cutoff decision is the reference's, bit for bit.
+// list slots per atom staged in LDS by the force kernel
+constexpr int STAGE = 16;
+
+// An atom's list index at slot s: from the LDS stage for its first STAGE slots
+// (all of them loaded at once before the pair loop, so the loop waits only on
+// neighbour gathers), from HBM beyond.
+__device__ __forceinline__ int list_at(const int (*stage)[BLK], const int *lp, int s)
+{
+    return s < STAGE ? stage[s][threadIdx.x] : lp[(size_t)s * TILE];
+}
+
+__device__ __forceinline__ void stage_list(int (*stage)[BLK], const int *lp, int nn, int self)
+{
+    int tmp[STAGE];
+#pragma unroll
+    for (int s = 0; s < STAGE; ++s) tmp[s] = s < nn ? lp[(size_t)s * TILE] : self;
+#pragma unroll
+    for (int s = 0; s < STAGE; ++s) stage[s][threadIdx.x] = tmp[s];
+}
+
 template <int FORM, int U>
 __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                               int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                              double &e)
+                                              double &e, int (*stage)[BLK])
 {
     const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
@@ -353,12 +373,13 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
     const double rc2 = tp.atrcut * tp.atrcut;
     const int nn = d.nn[i];
     const int *lp = d.list + list_slot(tp, rel);
+    stage_list(stage, lp, nn, i);
     fx = fy = fz = 0.0;
     for (int s0 = 0; s0 < nn; s0 += U) {
         int jj[U], tj[U];
         double xj[U], yj[U], zj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? lp[(size_t)(s0 + u) * TILE] : i;
+        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? list_at(stage, lp, s0 + u) : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             xj[u] = d.x[jj[u]];
@@ -420,7 +441,7 @@ __device__ __forceinline__ float sep_mixed(int ci, int cj, int nc, float side, f
 template <int FORM, bool SMALL, int U>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                               double &e)
+                                               double &e, int (*stage)[BLK])
 {
     const float4 pi = d.posf[i];
     int cxi, cyi, czi;
@@ -432,12 +453,13 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float rc2f = (float)(tp.atrcut * tp.atrcut);
     const int nn = d.nn[i];
     const int *lp = d.list + list_slot(tp, rel);
+    stage_list(stage, lp, nn, i);
     KahanF kx, ky, kz;
     for (int s0 = 0; s0 < nn; s0 += U) {
         int jj[U], tj[U];
         float4 pj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? lp[(size_t)(s0 + u) * TILE] : i;
+        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? list_at(stage, lp, s0 + u) : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             pj[u] = d.posf[jj[u]];
@@ -478,15 +500,15 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
 template <typename AccT, int FORM, int U>
 __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                          int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                         double &e)
+                                         double &e, int (*stage)[BLK])
 {
     if constexpr (sizeof(AccT) == 4) {
         if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
         else
-            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
     } else
-        pair_sum_fp64<FORM, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+        pair_sum_fp64<FORM, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
 }
 
 // LJ forces over the full neighbour list plus the integrator's fused
@@ -498,6 +520,7 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
     __shared__ double lds[4];
+    __shared__ int stage[STAGE][BLK];  // each thread's first list slots
     const int b = xcd_block(blockIdx.x, d.nblk);
     if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
@@ -510,12 +533,12 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
     const bool form1 = !MULTI || tp.accatom == ACC_1;
     double e = 0.0, eks = 0.0, ekfree = 0.0;
     if (act) {
-        const int ti = d.typ[i];
+        const int ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
         double fx, fy, fz;
         if (form1)
-            pair_sum<AccT, ACC_1, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+            pair_sum<AccT, ACC_1, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
         else
-            pair_sum<AccT, ACC_M, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e);
+            pair_sum<AccT, ACC_M, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
         // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:408-414
         double ax, ay, az;
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
@@ -585,7 +608,7 @@ __global__ __launch_bounds__(BLK) void k_kick_nh(DevPtrs d)
     const double dth = tp.dt / 2;
     double ax, ay, az;
     load_acc<AccT>(d, i, ax, ay, az);
-    const double den = nh_coupled(tp, d.typ[i]) ? 1.0 + dth * ts.psi : 1.0;
+    const double den = nh_coupled(tp, tp.nsorts > 1 ? d.typ[i] : 0) ? 1.0 + dth * ts.psi : 1.0;
     d.vx[i] = (d.vx[i] + dth * ax) / den;
     d.vy[i] = (d.vy[i] + dth * ay) / den;
     d.vz[i] = (d.vz[i] + dth * az) / den;
@@ -601,7 +624,7 @@ __global__ __launch_bounds__(BLK) void k_refresh_posf(DevPtrs d, int t)
     if (rel >= tp.natom) return;
     const int i = tp.abase + rel;
     int cx, cy, cz;
-    unpack_cell(d.posf[i].w, cx, cy, cz);
+    unpack_cell(__int_as_float(d.cellid[i]), cx, cy, cz);
     d.posf[i] = make_posf(d.x[i], d.y[i], d.z[i], cx, cy, cz, ts);
 }
 

@@ -87,7 +87,7 @@ EXPORTS = ("fcmd_device_count", "fcmd_error_text", "fcmd_create", "fcmd_destroy"
            "fcmd_get_scalars", "fcmd_neighbor", "fcmd_accatom", "fcmd_integrator",
            "fcmd_stoddard", "fcmd_halflist", "fcmd_run", "fcmd_init_random",
            "fcmd_last_run_ms", "fcmd_time_kernels", "fcmd_list_entries", "fcmd_set_tuning",
-           "fcmd_get_tuning", "fcmd_autotune")
+           "fcmd_get_tuning", "fcmd_autotune", "fcmd_size_counts", "fcmd_get_size_counts")
 
 
 def lib():
@@ -288,6 +288,18 @@ class System:
         check(lib().fcmd_run(self.h, int(nsteps), int(cluster_every), int(fpt_size),
                              _p(series, _PI), _p(fpt, _PI), int(bool(use_graph))))
         return series[:nsnap], fpt
+
+    def size_counts(self, nmax: int, warmup: int = 0):
+        """Start sizer-style cluster-size counting in later run() analyses
+        (helpers/sizer.f:59-103); nmax <= 0 stops it."""
+        check(lib().fcmd_size_counts(self.h, int(nmax), int(warmup)))
+
+    def get_size_counts(self, traj: int = 0):
+        """(counts[natom+1], stopped) of one trajectory."""
+        counts = np.zeros(self.natom[traj] + 1, dtype=np.int32)
+        stopped = np.zeros(1, dtype=np.int32)
+        check(lib().fcmd_get_size_counts(self.h, traj, _p(counts, _PI), _p(stopped, _PI)))
+        return counts, bool(stopped[0])
 
     def last_run_ms(self) -> float:
         v = C.c_double()

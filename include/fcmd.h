@@ -161,6 +161,18 @@ int fcmd_init_random(unsigned long long seed, int natom, const double *box, doub
                      double att, const double *mas, const int *api, double *x, double *y,
                      double *z, double *vx, double *vy, double *vz);
 
+/* Cluster-size probability counts (helpers/sizer.f:59-103, Stillinger
+ * histogram): from now on every analysis of fcmd_run at a step >= warmup adds
+ * csi(g) to counts(g) for g = 1 up to and including the first size >= nmax
+ * present in that analysis, after which the trajectory stops counting (sizer
+ * stops at its first cluster of nmax or more, also before warmup).  Resets
+ * the counts and the stop flags; nmax <= 0 switches counting off. */
+int fcmd_size_counts(fcmd_system *s, int nmax, int warmup);
+
+/* counts[natom+1] of trajectory traj (counts[0] = 0) and *stopped (may be
+ * NULL): 1 once it has met a cluster of nmax or more. */
+int fcmd_get_size_counts(fcmd_system *s, int traj, int *counts, int *stopped);
+
 /* Time of the last fcmd_run in milliseconds (HIP events on the run stream). */
 int fcmd_last_run_ms(fcmd_system *s, double *total_ms);
 
