@@ -343,7 +343,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.x2, N) A(d.y2, N) A(d.z2, N) A(d.vx2, N) A(d.vy2, N) A(d.vz2, N);
     if (s->mixed) {
         A(d.axf, N) A(d.ayf, N) A(d.azf, N) A(d.axf2, N) A(d.ayf2, N) A(d.azf2, N);
-        A(d.posf, N) A(d.cellid, N);
+        A(d.cellid, N);
     } else {
         A(d.ax, N) A(d.ay, N) A(d.az, N) A(d.ax2, N) A(d.ay2, N) A(d.az2, N);
     }
@@ -355,7 +355,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.wl_size, (long long)WL_LISTS * ntraj);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, s->g.ncblk);
-    A(d.nn, N) A(d.list, lbase);
+    A(d.nn, N) A(d.list, lbase) A(d.posf, N);
     A(d.edeg, N) A(d.edeg1, N) A(d.edges, N * EMAX) A(d.parent, N) A(d.parentf, N) A(d.liq, N);
     A(d.cs_tmp, N) A(d.cgs_tmp, N) A(d.csf_tmp, N) A(d.cek_tmp, N) A(d.cekf_tmp, N);
     A(d.liqo, N) A(d.rank, N) A(d.o_cl, N) A(d.o_cs, N) A(d.o_cgs, N) A(d.o_clf, N) A(d.o_ccf, N);

@@ -136,9 +136,10 @@ struct DevPtrs {
     // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE]
     int *nn;
     int *list;
-    // mixed path only (null otherwise): per slot, the FP32 offset of the atom
-    // from the centre of its cell at the last list build, and that cell packed
-    // in w (make_posf)
+    // per slot: the FP32 offset of the atom from the centre of its cell at the
+    // last list build, and that cell packed in w (make_posf).  Written at each
+    // list build (the list search reads it); on the mixed path also kept
+    // current by the drift (the force kernel's positions).
     float4 *posf;
     int *cellid;                     // mixed path: the packed cell of posf.w, alone (the drift reads it)
     // clusters: per slot

@@ -107,6 +107,16 @@ __global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
          w_ < e_; w_ += gridDim.x / NXCD)                                             \
         if (const int item = (list)[w_]; true)
 
+// One item per workgroup, the same XCD-aware split as FOR_ITEMS, for kernels
+// launched with at least as many workgroups as the list can hold (a loop over
+// items keeps loop-invariant values live across the body and costs registers
+// that a heavy body needs).  Workgroups past the list's length return.
+#define ONE_ITEM(list, k, item)                                                      \
+    const int n_ = d.wl_cnt[k], c_ = (n_ + NXCD - 1) / NXCD;                         \
+    const int w_ = (int)(blockIdx.x % NXCD) * c_ + (int)(blockIdx.x / NXCD);         \
+    if ((int)(blockIdx.x / NXCD) >= c_ || w_ >= n_) return;                          \
+    const int item = (list)[w_]
+
 // ---------------------------------------------------------------------------
 // cell_start = exclusive scan of cell_cnt within each trajectory: a scan of
 // each SCAN_CHUNK-cell chunk (k_scan_chunk), a scan of each trajectory's chunk
@@ -252,8 +262,9 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
     }
 }
 
-// gather every per-atom array into cell order (into the alternate buffers);
-// on the mixed path also write each atom's cell-offset record for its cell
+// gather every per-atom array into cell order (into the alternate buffers)
+// and write each atom's cell-offset record for its cell (on the mixed path
+// also the cell alone, for the drift)
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 {
@@ -264,13 +275,13 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
         const int i = b * BLK + threadIdx.x;
         const bool act = i - tp.abase < tp.natom;
         const int o = act ? d.sorted[i] : i;
-        if (sizeof(AccT) == 4 && act) {
+        if (act) {
             const TrajScalars &ts = d.ts[t];
             const int c = d.atom_cell[o];
             const int cx = c % ts.ncx, cy = (c / ts.ncx) % ts.ncy, cz = c / (ts.ncx * ts.ncy);
             const float4 r = make_posf(d.x[o], d.y[o], d.z[o], cx, cy, cz, ts);
             d.posf[i] = r;
-            d.cellid[i] = __float_as_int(r.w);
+            if (sizeof(AccT) == 4) d.cellid[i] = __float_as_int(r.w);
         }
         d.x2[i] = d.x[o];
         d.y2[i] = d.y[o];
@@ -293,11 +304,45 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 }
 
 // ---------------------------------------------------------------------------
-// The candidate search, on the permuted arrays (x2, oid2: cell order): every
-// atom j != i of the 27 (or fewer, without duplicates) cells around i's cell
-// whose reference r2 is within the list radius, in a fixed order (cells
-// z-major, then y, then x; atoms of one cell by reference number); sink(j,
-// cnt) is called for each.
+// The reference's pair test on the permuted FP64 positions: on wrapped
+// positions, d = x(i) - x(j) with i the lower reference atom number,
+// d = d + L - L*int(1.5 + d/L), r2 <= atrskin^2 (neighbor-pbc.f:79-84)
+// one coordinate of a separation from two cell-offset records (the cell
+// difference wrapped to a neighbour; below 3 cells a side, the rounding image)
+__device__ __forceinline__ float sep_cells(int ci, int cj, int nc, float side, float doff, float box,
+                                           float ibox)
+{
+    int dc = ci - cj;
+    dc += dc > 1 ? -nc : (dc < -1 ? nc : 0);
+    float s = (float)dc * side + doff;
+    if (nc < 3) s -= box * rintf(s * ibox);
+    return s;
+}
+
+__device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, const TrajParams &tp, int i, int j,
+                                                 double rs2)
+{
+    const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
+    const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
+    const bool fwd = d.oid2[i] < d.oid2[j];
+    const int a = fwd ? i : j, c = fwd ? j : i;
+    double dx = d.x2[a] - d.x2[c];
+    double dy = d.y2[a] - d.y2[c];
+    double dz = d.z2[a] - d.z2[c];
+    dx = dx + bx - bx * fint(1.5 + dx * ibx);
+    dy = dy + by - by * fint(1.5 + dy * iby);
+    dz = dz + bz - bz * fint(1.5 + dz * ibz);
+    return dx * dx + dy * dy + dz * dz <= rs2;
+}
+
+// ---------------------------------------------------------------------------
+// The candidate search, on the permuted slots (cell order): every atom j != i
+// of the 27 (or fewer, without duplicates) cells around i's cell whose
+// reference r2 is within the list radius, in a fixed order (cells z-major,
+// then y, then x; atoms of one cell by reference number); sink(j, cnt) is
+// called for each.  Candidates are read as their 16-byte cell-offset records
+// (one load, FP32 arithmetic); only the rare pair within 1e-4 of the radius
+// takes the reference's FP64 test.
 //
 // Atoms are sorted by cell, so the x-neighbouring cells of one (y, z) row
 // are one contiguous slot range, or two where the row wraps round the box.
@@ -309,15 +354,19 @@ template <int CU, typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                           int i, int &half, Sink sink)
 {
-    const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
-    const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
     const double rs2 = ts.rs2;
-    const double xi = d.x2[i], yi = d.y2[i], zi = d.z2[i];
-    const int oi = d.oid2[i];
+    // FP32 decision band: r2 computed from the cell-offset records is within
+    // ~2e-6 relative of the reference's r2 (offsets rounded once to FP32,
+    // |offset| <= a cell side), so outside +-1e-4 its verdict is the
+    // reference's; inside, the reference's FP64 test decides
+    const float in2 = (float)(rs2 * (1.0 - 1e-4)), out2 = (float)(rs2 * (1.0 + 1e-4));
+    const float4 pi = d.posf[i];
     const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
-    const int cx = cell_coord(xi, ts.cside[0], ncx);
-    const int cy = cell_coord(yi, ts.cside[1], ncy);
-    const int cz = cell_coord(zi, ts.cside[2], ncz);
+    int cx, cy, cz;
+    unpack_cell(pi.w, cx, cy, cz);
+    const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
+    const float bxf = (float)tp.box[0], byf = (float)tp.box[1], bzf = (float)tp.box[2];
+    const float ibxf = (float)tp.ibox[0], ibyf = (float)tp.ibox[1], ibzf = (float)tp.ibox[2];
     const int lx = ncx >= 3 ? -1 : 0, hx = ncx >= 3 ? 1 : ncx - 1;
     const int ly = ncy >= 3 ? -1 : 0, hy = ncy >= 3 ? 1 : ncy - 1;
     const int lz = ncz >= 3 ? -1 : 0, hz = ncz >= 3 ? 1 : ncz - 1;
@@ -366,40 +415,31 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
             a0[0] = a0[1], la[0] = la[1], b0[0] = b0[1], len[0] = len[1];
             a0[1] = a0[2], la[1] = la[2], b0[1] = b0[2], len[1] = len[2];
             for (int k0 = 0; k0 < n; k0 += CU) {
-                int jj[CU], oj[CU];
-                double xj[CU], yj[CU], zj[CU];
+                int jj[CU];
+                float4 pj[CU];
 #pragma unroll
                 for (int u = 0; u < CU; ++u) {
                     const int k = min(k0 + u, n - 1);
                     jj[u] = k < na ? base_a + k : base_b + k;
-                    oj[u] = d.oid2[jj[u]];
-                    xj[u] = d.x2[jj[u]];
-                    yj[u] = d.y2[jj[u]];
-                    zj[u] = d.z2[jj[u]];
+                    pj[u] = d.posf[jj[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < CU; ++u) {
                     if (k0 + u >= n) break;
                     const int j = jj[u];
                     if (j == i) continue;
-                    double dx, dy, dz;
-                    if (oi < oj[u]) {
-                        dx = xi - xj[u];
-                        dy = yi - yj[u];
-                        dz = zi - zj[u];
-                    } else {
-                        dx = xj[u] - xi;
-                        dy = yj[u] - yi;
-                        dz = zj[u] - zi;
-                    }
-                    dx = dx + bx - bx * fint(1.5 + dx * ibx);
-                    dy = dy + by - by * fint(1.5 + dy * iby);
-                    dz = dz + bz - bz * fint(1.5 + dz * ibz);
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 <= rs2) {
+                    int cxj, cyj, czj;
+                    unpack_cell(pj[u].w, cxj, cyj, czj);
+                    const float fx = sep_cells(cx, cxj, ncx, sx, pi.x - pj[u].x, bxf, ibxf);
+                    const float fy = sep_cells(cy, cyj, ncy, sy, pi.y - pj[u].y, byf, ibyf);
+                    const float fz = sep_cells(cz, czj, ncz, sz, pi.z - pj[u].z, bzf, ibzf);
+                    const float r2f = fx * fx + fy * fy + fz * fz;
+                    bool in = r2f <= in2;
+                    if (!in && r2f <= out2) in = reference_pair_test(d, tp, i, j, rs2);
+                    if (in) {
                         sink(j, cnt);
                         ++cnt;
-                        half += (oi < oj[u]);
+                        half += (i < j);
                     }
                 }
             }
@@ -417,9 +457,11 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
 template <int CU>
 __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
 {
-    __shared__ int ldsi[4];
-    FOR_ITEMS(d.wl_blk, WL_BLK, b)
+    __shared__ int red[2];
+    ONE_ITEM(d.wl_blk, WL_BLK, b);
     {
+        if (threadIdx.x == 0) red[0] = red[1] = 0;
+        __syncthreads();
         const int t = d.blk_traj[b];
         const TrajParams &tp = d.tp[t];
         TrajScalars &ts = d.ts[t];
@@ -438,10 +480,15 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
             cnt = width;
         }
         d.nn[i] = cnt;
-        const int w = wave_max_i(cnt);
-        if ((threadIdx.x & (TILE - 1)) == 0 && w > 0) atomicMax(&ts.nlist, w);
-        const int h = block_sum_i(half, ldsi);
-        if (threadIdx.x == 0 && h) atomicAdd(&ts.npairs, h);
+        // block max and sum through LDS atomics (integers: order-free), which
+        // keep no shuffle lane addresses live across the search
+        atomicMax(&red[0], cnt);
+        atomicAdd(&red[1], half);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (red[0] > 0) atomicMax(&ts.nlist, red[0]);
+            if (red[1]) atomicAdd(&ts.npairs, red[1]);
+        }
         // copy back: every slot of the block takes its permuted arrays (no
         // thread of this kernel reads the primary ones)
         d.x[i] = d.x2[i];
@@ -484,10 +531,12 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
         hipLaunchKernelGGL(k_permute<float>, ga, blk, 0, s, d);
     else
         hipLaunchKernelGGL(k_permute<double>, ga, blk, 0, s, d);
+    // one workgroup per possible item: the search needs the registers
+    const dim3 gall(xcd_grid(g.nblk));
     if (g.nbr_cu == 4)
-        hipLaunchKernelGGL(k_nbr<4>, ga, blk, 0, s, d);
+        hipLaunchKernelGGL(k_nbr<4>, gall, blk, 0, s, d);
     else
-        hipLaunchKernelGGL(k_nbr<2>, ga, blk, 0, s, d);
+        hipLaunchKernelGGL(k_nbr<2>, gall, blk, 0, s, d);
 }
 
 }  // namespace fcmd

@@ -630,7 +630,9 @@ __global__ __launch_bounds__(BLK) void k_refresh_posf(DevPtrs d, int t)
 
 void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s)
 {
-    if (d.posf) hipLaunchKernelGGL(k_refresh_posf, dim3((natom + BLK - 1) / BLK), dim3(BLK), 0, s, d, t);
+    // the mixed path only: elsewhere the records are rewritten at each list
+    // build before anything reads them
+    if (d.cellid) hipLaunchKernelGGL(k_refresh_posf, dim3((natom + BLK - 1) / BLK), dim3(BLK), 0, s, d, t);
 }
 
 // neighbor(step) called directly (init-cube1.f:113, read-rst.f:63): only
