@@ -101,6 +101,7 @@ struct TrajScalars {
     double dmmsr;                    // dmms when the last rebuild was decided
     double rs2;                      // list radius^2 of the current list
     double cside[3];                 // cell side used by the current list
+    double icside[3];                // 1 / cside
     int dmmst, dmmi, step, rebuild, err, npairs, nrebuild, nlist;
     int ncx, ncy, ncz, pad0;
     int nh_iters;
@@ -131,7 +132,8 @@ struct DevPtrs {
     int *wl_blk, *wl_cchk, *wl_traj, *wl_cnt;
     int *wl_size;                    // WL_LISTS per trajectory: its item counts
     // cells
-    int *cell_cnt, *cell_start, *atom_cell, *atom_slot, *sorted;
+    int *cell_cnt, *cell_start, *sorted;
+    int *atom_cell, *atom_slot;      // per slot at binning: packed cell (CELL_BITS each), slot in cell
     long long *scan_tmp_ll;          // chunk totals for segmented scans
     // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE]
     int *nn;

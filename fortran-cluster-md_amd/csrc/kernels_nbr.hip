@@ -186,10 +186,27 @@ __global__ __launch_bounds__(BLK) void k_zero_cells(DevPtrs d)
     }
 }
 
-__device__ __forceinline__ int cell_coord(double p, double side, int n)
+// cell of a wrapped coordinate: p * (1/side), whose rounding is far below
+// the margin the cell side keeps over the list radius
+__device__ __forceinline__ int cell_coord(double p, double inv_side, int n)
 {
-    int c = (int)(p / side);
+    int c = (int)(p * inv_side);
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+// dmod(dmod(x,L)+L,L), neighbor-pbc.f:62-66.  For |x| < L the inner dmod is x
+// itself and the outer one takes a = x+L in [0, 2L] down by L at most twice,
+// exactly (Sterbenz), so the usual case needs no fmod; the result is the same
+// double either way.
+__device__ __forceinline__ double wrap_box(double x, double L)
+{
+    if (fabs(x) < L) {
+        const double a = x + L;
+        if (a < L) return a;
+        const double r = a - L;
+        return r < L ? r : r - L;
+    }
+    return fmod(fmod(x, L) + L, L);
 }
 
 // wrap in place (neighbor-pbc.f:62-66) and bin
@@ -201,18 +218,17 @@ __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
         const TrajScalars &ts = d.ts[d.blk_traj[b]];
         const int i = b * BLK + threadIdx.x;
         if (i - tp.abase < tp.natom) {
-            const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
-            const double x = fmod(fmod(d.x[i], bx) + bx, bx);
-            const double y = fmod(fmod(d.y[i], by) + by, by);
-            const double z = fmod(fmod(d.z[i], bz) + bz, bz);
+            const double x = wrap_box(d.x[i], tp.box[0]);
+            const double y = wrap_box(d.y[i], tp.box[1]);
+            const double z = wrap_box(d.z[i], tp.box[2]);
             d.x[i] = x;
             d.y[i] = y;
             d.z[i] = z;
-            const int cx = cell_coord(x, ts.cside[0], ts.ncx);
-            const int cy = cell_coord(y, ts.cside[1], ts.ncy);
-            const int cz = cell_coord(z, ts.cside[2], ts.ncz);
+            const int cx = cell_coord(x, ts.icside[0], ts.ncx);
+            const int cy = cell_coord(y, ts.icside[1], ts.ncy);
+            const int cz = cell_coord(z, ts.icside[2], ts.ncz);
             const int c = (cz * ts.ncy + cy) * ts.ncx + cx;
-            d.atom_cell[i] = c;
+            d.atom_cell[i] = cx | (cy << CELL_BITS) | (cz << (2 * CELL_BITS));  // packed
             d.atom_slot[i] = atomicAdd(d.cell_cnt + tp.cbase + c, 1);
         }
     }
@@ -225,7 +241,10 @@ __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
         const TrajParams &tp = d.tp[d.blk_traj[b]];
         const int i = b * BLK + threadIdx.x;
         if (i - tp.abase < tp.natom) {
-            const int c = tp.cbase + d.atom_cell[i];
+            const TrajScalars &ts = d.ts[d.blk_traj[b]];
+            const int pc = d.atom_cell[i];
+            const int c = tp.cbase + ((pc >> (2 * CELL_BITS)) * ts.ncy + ((pc >> CELL_BITS) & CELL_MAX)) * ts.ncx +
+                          (pc & CELL_MAX);
             const int start = d.cell_start[c] + (int)d.scan_tmp_ll[c / SCAN_CHUNK];
             d.sorted[tp.abase + start + d.atom_slot[i]] = i;
         }
@@ -235,14 +254,15 @@ __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
 // insertion sort of each cell's atoms by reference atom number, so the
 // order (and every later floating-point sum) does not depend on atomics; also
 // writes each cell's final start
-__global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
+__global__ __launch_bounds__(SCAN_CHUNK) void k_cellsort(DevPtrs d)
 {
     FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
         const TrajParams &tp = d.tp[d.cblk_traj[c]];
         const int add = (int)d.scan_tmp_ll[c];
-        for (int k = 0; k < 4; ++k) {
-            const int cell = c * SCAN_CHUNK + threadIdx.x * 4 + k;
+        {
+            // one cell per thread (SCAN_CHUNK threads per chunk)
+            const int cell = c * SCAN_CHUNK + threadIdx.x;
             const int start = d.cell_start[cell] + add;
             d.cell_start[cell] = start;
             const int n = d.cell_cnt[cell];
@@ -277,8 +297,8 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
         const int o = act ? d.sorted[i] : i;
         if (act) {
             const TrajScalars &ts = d.ts[t];
-            const int c = d.atom_cell[o];
-            const int cx = c % ts.ncx, cy = (c / ts.ncx) % ts.ncy, cz = c / (ts.ncx * ts.ncy);
+            int cx, cy, cz;
+            unpack_cell(__int_as_float(d.atom_cell[o]), cx, cy, cz);
             const float4 r = make_posf(d.x[o], d.y[o], d.z[o], cx, cy, cz, ts);
             d.posf[i] = r;
             if (sizeof(AccT) == 4) d.cellid[i] = __float_as_int(r.w);
@@ -526,7 +546,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, ga, blk, 0, s, d);
-    hipLaunchKernelGGL(k_cellsort, gc, blk, 0, s, d);
+    hipLaunchKernelGGL(k_cellsort, gc, dim3(SCAN_CHUNK), 0, s, d);
     if (mixed)
         hipLaunchKernelGGL(k_permute<float>, ga, blk, 0, s, d);
     else

@@ -29,6 +29,7 @@ __device__ void set_cell_grid(const TrajParams &tp, TrajScalars &ts)
         n[k] = (int)floor(tp.box[k] / (ts.atrskin * (1.0 + 1e-9)));
         n[k] = n[k] < 1 ? 1 : (n[k] > CELL_MAX ? CELL_MAX : n[k]);
         ts.cside[k] = tp.box[k] / (double)n[k];
+        ts.icside[k] = 1.0 / ts.cside[k];
     }
     if ((long long)n[0] * n[1] * n[2] > (long long)tp.ncpad) {
         ts.err = ERR_CELLS;
