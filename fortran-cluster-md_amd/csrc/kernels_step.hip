@@ -178,9 +178,18 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
     double d2 = 0.0, sv = 0.0;
     if (act) {
         const double dt = tp.dt;
-        double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
+        double vx = __builtin_nontemporal_load(d.vx + i), vy = __builtin_nontemporal_load(d.vy + i),
+               vz = __builtin_nontemporal_load(d.vz + i);
         double ax, ay, az;
-        load_acc<AccT>(d, i, ax, ay, az);
+        if (sizeof(AccT) == 4) {
+            ax = __builtin_nontemporal_load(d.axf + i);
+            ay = __builtin_nontemporal_load(d.ayf + i);
+            az = __builtin_nontemporal_load(d.azf + i);
+        } else {
+            ax = __builtin_nontemporal_load(d.ax + i);
+            ay = __builtin_nontemporal_load(d.ay + i);
+            az = __builtin_nontemporal_load(d.az + i);
+        }
         double dx, dy, dz;
         if (tp.integrator == INT_NH) {
             const double dt2 = dt * dt / 2, dth = dt / 2;
@@ -214,18 +223,19 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
             vy = vy + 0.5 * ay * dt;
             vz = vz + 0.5 * az * dt;
         }
-        const double xn = d.x[i] + dx, yn = d.y[i] + dy, zn = d.z[i] + dz;
-        d.x[i] = xn;
-        d.y[i] = yn;
-        d.z[i] = zn;
+        const double xn = __builtin_nontemporal_load(d.x + i) + dx, yn = __builtin_nontemporal_load(d.y + i) + dy,
+                     zn = __builtin_nontemporal_load(d.z + i) + dz;
+        __builtin_nontemporal_store(xn, d.x + i);
+        __builtin_nontemporal_store(yn, d.y + i);
+        __builtin_nontemporal_store(zn, d.z + i);
         if (sizeof(AccT) == 4) {
             int cx, cy, cz;
-            unpack_cell(__int_as_float(d.cellid[i]), cx, cy, cz);
+            unpack_cell(__int_as_float(__builtin_nontemporal_load(d.cellid + i)), cx, cy, cz);
             d.posf[i] = make_posf(xn, yn, zn, cx, cy, cz, ts);
         }
-        d.vx[i] = vx;
-        d.vy[i] = vy;
-        d.vz[i] = vz;
+        __builtin_nontemporal_store(vx, d.vx + i);
+        __builtin_nontemporal_store(vy, d.vy + i);
+        __builtin_nontemporal_store(vz, d.vz + i);
         d2 = dx * dx + dy * dy + dz * dz;
     }
     const double bmax = block_max(d2, lds);
@@ -344,19 +354,24 @@ __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
 // list slots per atom staged in LDS by the force kernel
 constexpr int STAGE = 16;
 
+// Streaming data of the force kernel (the atom's own count, list slots,
+// velocities and the accelerations it writes) is read or written once per
+// step: nontemporal accesses keep it from displacing the neighbours'
+// records, which the same and nearby wavefronts gather again from L2.
+
 // An atom's list index at slot s: from the LDS stage for its first STAGE slots
 // (all of them loaded at once before the pair loop, so the loop waits only on
 // neighbour gathers), from HBM beyond.
 __device__ __forceinline__ int list_at(const int (*stage)[BLK], const int *lp, int s)
 {
-    return s < STAGE ? stage[s][threadIdx.x] : lp[(size_t)s * TILE];
+    return s < STAGE ? stage[s][threadIdx.x] : __builtin_nontemporal_load(lp + (size_t)s * TILE);
 }
 
 __device__ __forceinline__ void stage_list(int (*stage)[BLK], const int *lp, int nn, int self)
 {
     int tmp[STAGE];
 #pragma unroll
-    for (int s = 0; s < STAGE; ++s) tmp[s] = s < nn ? lp[(size_t)s * TILE] : self;
+    for (int s = 0; s < STAGE; ++s) tmp[s] = s < nn ? __builtin_nontemporal_load(lp + (size_t)s * TILE) : self;
 #pragma unroll
     for (int s = 0; s < STAGE; ++s) stage[s][threadIdx.x] = tmp[s];
 }
@@ -372,7 +387,7 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
     const double bf1 = 2.0 + fint(ts.atrskin / tp.atr);
     const double bf2 = bf1 + 0.5;
     const double rc2 = tp.atrcut * tp.atrcut;
-    const int nn = d.nn[i];
+    const int nn = __builtin_nontemporal_load(d.nn + i);
     const int *lp = d.list + list_slot(tp, rel);
     stage_list(stage, lp, nn, i);
     fx = fy = fz = 0.0;
@@ -452,7 +467,7 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
     const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
     const float rc2f = (float)(tp.atrcut * tp.atrcut);
-    const int nn = d.nn[i];
+    const int nn = __builtin_nontemporal_load(d.nn + i);
     const int *lp = d.list + list_slot(tp, rel);
     stage_list(stage, lp, nn, i);
     KahanF kx, ky, kz;
@@ -555,11 +570,24 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
             ay = ay0 + fy * iatm;
             az = az0 + fz * iatm;
         }
-        store_acc<AccT>(d, i, ax, ay, az);
-        if (MIXED) load_acc<AccT>(d, i, ax, ay, az);  // integrate what is stored
+        if (MIXED) {
+            // integrate what is stored
+            const float axs = (float)ax, ays = (float)ay, azs = (float)az;
+            __builtin_nontemporal_store(axs, d.axf + i);
+            __builtin_nontemporal_store(ays, d.ayf + i);
+            __builtin_nontemporal_store(azs, d.azf + i);
+            ax = axs;
+            ay = ays;
+            az = azs;
+        } else {
+            __builtin_nontemporal_store(ax, d.ax + i);
+            __builtin_nontemporal_store(ay, d.ay + i);
+            __builtin_nontemporal_store(az, d.az + i);
+        }
         if (mode == FORCE_STEP) {
             const double dt = tp.dt;
-            double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
+            double vx = __builtin_nontemporal_load(d.vx + i), vy = __builtin_nontemporal_load(d.vy + i),
+                   vz = __builtin_nontemporal_load(d.vz + i);
             if (tp.integrator == INT_NH) {
                 const double dth = dt / 2;
                 const double wx = vx + dth * ax, wy = vy + dth * ay, wz = vz + dth * az;
