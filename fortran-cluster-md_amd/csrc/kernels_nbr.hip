@@ -492,7 +492,7 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         if (rel < tp.natom) {
             int *out = d.list + list_slot(tp, rel);
             cnt = candidates<CU>(d, tp, ts, i, half, [&](int j, int k) {
-                if (k < width) out[(size_t)k * TILE] = j;
+                if (k < width) __builtin_nontemporal_store(j, out + (size_t)k * TILE);
             });
         }
         if (cnt > width) {
@@ -511,22 +511,25 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         }
         // copy back: every slot of the block takes its permuted arrays (no
         // thread of this kernel reads the primary ones)
-        d.x[i] = d.x2[i];
-        d.y[i] = d.y2[i];
-        d.z[i] = d.z2[i];
-        d.vx[i] = d.vx2[i];
-        d.vy[i] = d.vy2[i];
-        d.vz[i] = d.vz2[i];
+        // (streamed: nontemporal)
+#define FCMD_MOVE(a) __builtin_nontemporal_store(__builtin_nontemporal_load(d.a##2 + i), d.a + i)
+        FCMD_MOVE(x);
+        FCMD_MOVE(y);
+        FCMD_MOVE(z);
+        FCMD_MOVE(vx);
+        FCMD_MOVE(vy);
+        FCMD_MOVE(vz);
         if (d.axf) {
-            d.axf[i] = d.axf2[i];
-            d.ayf[i] = d.ayf2[i];
-            d.azf[i] = d.azf2[i];
+            FCMD_MOVE(axf);
+            FCMD_MOVE(ayf);
+            FCMD_MOVE(azf);
         } else {
-            d.ax[i] = d.ax2[i];
-            d.ay[i] = d.ay2[i];
-            d.az[i] = d.az2[i];
+            FCMD_MOVE(ax);
+            FCMD_MOVE(ay);
+            FCMD_MOVE(az);
         }
-        d.typ[i] = d.typ2[i];
+        FCMD_MOVE(typ);
+#undef FCMD_MOVE
         const int o = d.oid2[i];
         d.oid[i] = o;
         if (rel < tp.natom) d.inv[tp.abase + o - 1] = i;
