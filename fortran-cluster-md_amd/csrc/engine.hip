@@ -299,7 +299,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         const double smin = std::min(p.atrskin, p.atrcut) * (1.0 - 1e-6);
         long long nc = 1;
         for (int d = 0; d < 3; ++d) nc *= std::max<long long>(1, (long long)std::floor(p.box[d] / smin));
-        const long long ncpad = roundup(nc, SCAN_CHUNK);
+        const long long ncpad = roundup(nc + 1, SCAN_CHUNK);  // + the end cell past the grid
         t.abase = (int)abase;
         t.npad = (int)npad;
         t.bfirst = (int)(abase / BLK);

@@ -8,10 +8,16 @@
 // which no trajectory rebuilds costs a few empty launches, and the sequence
 // can be captured in a graph because no host decision is involved.
 //
-//   wl_fill -> zero_cells -> bin (wrap + cell + slot) -> scan (cell counts)
-//   -> scatter -> cellsort (deterministic order inside a cell) -> permute
-//   (into the alternate buffers) -> nbr (search + list in one pass, then the
-//   copy back)
+//   wl_fill -> bin (wrap + cell + slot) -> scan (cell counts, which it
+//   clears for the next rebuild) -> scatter -> cellsort (deterministic order
+//   inside a cell) -> permute (into the alternate buffers) -> nbr (search +
+//   list in one pass, then the copy back)
+//
+// Cell counts are zero outside the bin -> scan window (zeroed at allocation;
+// the scan clears every count of the trajectory's cell range after reading
+// it).  After the cellsort, cell_start holds trajectory-relative starts for
+// every cell of the range; a cell's atoms are [start[c], start[c + 1]), and
+// the range keeps at least one cell past the grid, whose start is natom.
 //
 // The pair test is the reference's: on wrapped positions,
 //   d = x(i) - x(j);  d = d + L - L*int(1.5 + d/L);  r2 <= atrskin^2
@@ -119,9 +125,10 @@ __global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
 
 // ---------------------------------------------------------------------------
 // cell_start = exclusive scan of cell_cnt within each trajectory: a scan of
-// each SCAN_CHUNK-cell chunk (k_scan_chunk), a scan of each trajectory's chunk
-// totals (k_scan_totals); the chunk offset is added where a start is first
-// used (k_scatter reads start + offset, k_cellsort writes the final start)
+// each SCAN_CHUNK-cell chunk (k_scan_chunk, which also clears the counts), a
+// scan of each trajectory's chunk totals (k_scan_totals); the chunk offset is
+// added where a start is first used (k_scatter reads start + offset,
+// k_cellsort writes the final start)
 __global__ __launch_bounds__(BLK) void k_scan_chunk(DevPtrs d)
 {
     __shared__ long long lds[4];
@@ -134,6 +141,7 @@ __global__ __launch_bounds__(BLK) void k_scan_chunk(DevPtrs d)
             v[k] = d.cell_cnt[base + k];
             s += v[k];
         }
+        for (int k = 0; k < 4; ++k) d.cell_cnt[base + k] = 0;
         long long total;
         long long pre = block_excl_scan(s, lds, &total);
         for (int k = 0; k < 4; ++k) {
@@ -177,15 +185,6 @@ __global__ __launch_bounds__(BLK) void k_scan_totals(DevPtrs d)
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLK) void k_zero_cells(DevPtrs d)
-{
-    FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
-    {
-        const int base = c * SCAN_CHUNK + threadIdx.x * 4;
-        for (int k = 0; k < 4; ++k) d.cell_cnt[base + k] = 0;
-    }
-}
-
 // cell of a wrapped coordinate: p * (1/side), whose rounding is far below
 // the margin the cell side keeps over the list radius
 __device__ __forceinline__ int cell_coord(double p, double inv_side, int n)
@@ -212,7 +211,7 @@ __device__ __forceinline__ double wrap_box(double x, double L)
 // wrap in place (neighbor-pbc.f:62-66) and bin
 __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
 {
-    FOR_ITEMS(d.wl_blk, WL_BLK, b)
+    ONE_ITEM(d.wl_blk, WL_BLK, b);
     {
         const TrajParams &tp = d.tp[d.blk_traj[b]];
         const TrajScalars &ts = d.ts[d.blk_traj[b]];
@@ -236,7 +235,7 @@ __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
 
 __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
 {
-    FOR_ITEMS(d.wl_blk, WL_BLK, b)
+    ONE_ITEM(d.wl_blk, WL_BLK, b);
     {
         const TrajParams &tp = d.tp[d.blk_traj[b]];
         const int i = b * BLK + threadIdx.x;
@@ -251,23 +250,57 @@ __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
     }
 }
 
-// insertion sort of each cell's atoms by reference atom number, so the
-// order (and every later floating-point sum) does not depend on atomics; also
-// writes each cell's final start
+// compare-exchange of (key, value) pairs by key
+__device__ __forceinline__ void cmp_swap(int &ka, int &va, int &kb, int &vb)
+{
+    if (ka > kb) {
+        const int k = ka, v = va;
+        ka = kb, va = vb;
+        kb = k, vb = v;
+    }
+}
+
+// cells of up to this many atoms are sorted in registers
+constexpr int CS_REG = 8;
+
+// sort of each cell's atoms by reference atom number, so the order (and every
+// later floating-point sum) does not depend on atomics; also writes each
+// cell's final start.  One cell per thread (SCAN_CHUNK threads per chunk).
+// A cell of up to CS_REG atoms loads its members and their numbers at once
+// and sorts them in registers (odd-even transposition, CS_REG phases);
+// a larger one is sorted in place.
 __global__ __launch_bounds__(SCAN_CHUNK) void k_cellsort(DevPtrs d)
 {
     FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
         const TrajParams &tp = d.tp[d.cblk_traj[c]];
         const int add = (int)d.scan_tmp_ll[c];
-        {
-            // one cell per thread (SCAN_CHUNK threads per chunk)
-            const int cell = c * SCAN_CHUNK + threadIdx.x;
-            const int start = d.cell_start[cell] + add;
-            d.cell_start[cell] = start;
-            const int n = d.cell_cnt[cell];
-            if (n < 2) continue;
-            int *a = d.sorted + tp.abase + start;
+        const int cell = c * SCAN_CHUNK + threadIdx.x;
+        // chunk-relative start and end (the next cell's start; for the
+        // chunk's last cell, the next chunk's offset, or natom)
+        const int s0 = d.cell_start[cell];
+        const int e0 = threadIdx.x + 1 < SCAN_CHUNK ? d.cell_start[cell + 1]
+                       : (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
+        __syncthreads();  // every chunk-relative start is read before any is replaced
+        const int start = s0 + add;
+        d.cell_start[cell] = start;
+        const int n = e0 - s0;
+        if (n < 2) continue;
+        int *a = d.sorted + tp.abase + start;
+        if (n <= CS_REG) {
+            int v[CS_REG], key[CS_REG];
+#pragma unroll
+            for (int p = 0; p < CS_REG; ++p) v[p] = p < n ? a[p] : 0;
+#pragma unroll
+            for (int p = 0; p < CS_REG; ++p) key[p] = p < n ? d.oid[v[p]] : INT_MAX;
+#pragma unroll
+            for (int ph = 0; ph < CS_REG; ++ph)
+#pragma unroll
+                for (int p = ph & 1; p + 1 < CS_REG; p += 2) cmp_swap(key[p], v[p], key[p + 1], v[p + 1]);
+#pragma unroll
+            for (int p = 0; p < CS_REG; ++p)
+                if (p < n) a[p] = v[p];
+        } else {
             for (int p = 1; p < n; ++p) {
                 const int v = a[p];
                 const int key = d.oid[v];
@@ -288,7 +321,7 @@ __global__ __launch_bounds__(SCAN_CHUNK) void k_cellsort(DevPtrs d)
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 {
-    FOR_ITEMS(d.wl_blk, WL_BLK, b)
+    ONE_ITEM(d.wl_blk, WL_BLK, b);
     {
         const int t = d.blk_traj[b];
         const TrajParams &tp = d.tp[t];
@@ -367,9 +400,10 @@ __device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, const Traj
 // Atoms are sorted by cell, so the x-neighbouring cells of one (y, z) row
 // are one contiguous slot range, or two where the row wraps round the box.
 // The bounds of a z-plane's rows are loaded together before its first
-// candidate; each row is then walked as one sequence (its second range
-// following the first), CU candidates (a tunable) per iteration with their
-// loads issued together.  The visiting order is the same as cell by cell.
+// candidate; the plane's rows are then walked as one sequence (each row's
+// second range following its first), CU candidates (a tunable) per iteration
+// with their loads issued together.  The visiting order is the same as cell
+// by cell.
 template <int CU, typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                           int i, int &half, Sink sink)
@@ -401,66 +435,67 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
         xa1 = ncx - 1;
     }
     const int *cstart = d.cell_start + tp.cbase;
-    const int *ccnt = d.cell_cnt + tp.cbase;
     int cnt = 0;
 #pragma unroll 1
     for (int oz = lz; oz <= hz; ++oz) {
         int zz = cz + oz;
         zz += (zz < 0) ? ncz : (zz >= ncz ? -ncz : 0);
-        // per row: first range [a0, a0 + la), then [b0, b0 + (len - la))
-        int a0[3], la[3], b0[3], len[3];
+        // per row q: first range [A[q], A[q] + la[q]), then the rest from
+        // B[q] + la[q] (absolute slots); e0, e1, n: the plane's row ends
+        int A[3], la[3], B[3], len[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int oy = ly + q;
-            a0[q] = la[q] = b0[q] = len[q] = 0;
+            A[q] = la[q] = B[q] = len[q] = 0;
             if (oy <= hy) {
                 int yy = cy + oy;
                 yy += (yy < 0) ? ncy : (yy >= ncy ? -ncy : 0);
                 const int row = (zz * ncy + yy) * ncx;
-                a0[q] = cstart[row + xa0];
-                la[q] = cstart[row + xa1] + ccnt[row + xa1] - a0[q];
+                const int a0 = cstart[row + xa0];
+                la[q] = cstart[row + xa1 + 1] - a0;
                 len[q] = la[q];
+                A[q] = tp.abase + a0;
                 if (xb1 >= xb0) {
-                    b0[q] = cstart[row + xb0];
-                    len[q] += cstart[row + xb1] + ccnt[row + xb1] - b0[q];
+                    const int b0 = cstart[row + xb0];
+                    len[q] += cstart[row + xb1 + 1] - b0;
+                    B[q] = tp.abase + b0 - la[q];
                 }
             }
         }
-        // rows one at a time (not unrolled: the row's bounds are shifted into
-        // slot 0, so the arrays stay in registers)
-#pragma unroll 1
-        for (int q = 0; q < 3; ++q) {
-            const int n = len[0], na = la[0];
-            const int base_a = tp.abase + a0[0], base_b = tp.abase + b0[0] - na;
-            a0[0] = a0[1], la[0] = la[1], b0[0] = b0[1], len[0] = len[1];
-            a0[1] = a0[2], la[1] = la[2], b0[1] = b0[2], len[1] = len[2];
-            for (int k0 = 0; k0 < n; k0 += CU) {
-                int jj[CU];
-                float4 pj[CU];
+        // the plane's three rows walked as one sequence (lanes whose rows
+        // differ in length stay busy across row ends)
+        const int e0 = len[0], e1 = e0 + len[1], n = e1 + len[2];
+        for (int k0 = 0; k0 < n; k0 += CU) {
+            int jj[CU];
+            float4 pj[CU];
 #pragma unroll
-                for (int u = 0; u < CU; ++u) {
-                    const int k = min(k0 + u, n - 1);
-                    jj[u] = k < na ? base_a + k : base_b + k;
-                    pj[u] = d.posf[jj[u]];
-                }
+            for (int u = 0; u < CU; ++u) {
+                const int k = min(k0 + u, n - 1);
+                const bool r0 = k < e0, r1 = k < e1;
+                const int kr = r0 ? k : (r1 ? k - e0 : k - e1);
+                const int na = r0 ? la[0] : (r1 ? la[1] : la[2]);
+                const int ba = r0 ? A[0] : (r1 ? A[1] : A[2]);
+                const int bb = r0 ? B[0] : (r1 ? B[1] : B[2]);
+                jj[u] = kr < na ? ba + kr : bb + kr;
+                pj[u] = d.posf[jj[u]];
+            }
 #pragma unroll
-                for (int u = 0; u < CU; ++u) {
-                    if (k0 + u >= n) break;
-                    const int j = jj[u];
-                    if (j == i) continue;
-                    int cxj, cyj, czj;
-                    unpack_cell(pj[u].w, cxj, cyj, czj);
-                    const float fx = sep_cells(cx, cxj, ncx, sx, pi.x - pj[u].x, bxf, ibxf);
-                    const float fy = sep_cells(cy, cyj, ncy, sy, pi.y - pj[u].y, byf, ibyf);
-                    const float fz = sep_cells(cz, czj, ncz, sz, pi.z - pj[u].z, bzf, ibzf);
-                    const float r2f = fx * fx + fy * fy + fz * fz;
-                    bool in = r2f <= in2;
-                    if (!in && r2f <= out2) in = reference_pair_test(d, tp, i, j, rs2);
-                    if (in) {
-                        sink(j, cnt);
-                        ++cnt;
-                        half += (i < j);
-                    }
+            for (int u = 0; u < CU; ++u) {
+                if (k0 + u >= n) break;
+                const int j = jj[u];
+                if (j == i) continue;
+                int cxj, cyj, czj;
+                unpack_cell(pj[u].w, cxj, cyj, czj);
+                const float fx = sep_cells(cx, cxj, ncx, sx, pi.x - pj[u].x, bxf, ibxf);
+                const float fy = sep_cells(cy, cyj, ncy, sy, pi.y - pj[u].y, byf, ibyf);
+                const float fz = sep_cells(cz, czj, ncz, sz, pi.z - pj[u].z, bzf, ibzf);
+                const float r2f = fx * fx + fy * fy + fz * fz;
+                bool in = r2f <= in2;
+                if (!in && r2f <= out2) in = reference_pair_test(d, tp, i, j, rs2);
+                if (in) {
+                    sink(j, cnt);
+                    ++cnt;
+                    half += (i < j);
                 }
             }
         }
@@ -541,21 +576,22 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
 {
     const dim3 blk(BLK);
     const int wg = g.wl_grid;  // workgroups per rebuild kernel (a tunable)
-    const dim3 ga(xcd_grid(std::min(g.nblk, wg))), gc(xcd_grid(std::min(g.ncblk, wg)));
+    const dim3 gc(xcd_grid(std::min(g.ncblk, wg)));
     const dim3 gj(xcd_grid(std::min(g.ntraj, wg)));
     hipLaunchKernelGGL(k_wl_fill, dim3(g.ntraj), blk, 0, s, d, g.ntraj);
-    hipLaunchKernelGGL(k_zero_cells, gc, blk, 0, s, d);
-    hipLaunchKernelGGL(k_bin, ga, blk, 0, s, d);
+    // one workgroup per possible atom block where a block's work is a short
+    // chain of dependent accesses (no loop: every block starts at once)
+    const dim3 gall(xcd_grid(g.nblk));
+    hipLaunchKernelGGL(k_bin, gall, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scatter, ga, blk, 0, s, d);
+    hipLaunchKernelGGL(k_scatter, gall, blk, 0, s, d);
     hipLaunchKernelGGL(k_cellsort, gc, dim3(SCAN_CHUNK), 0, s, d);
     if (mixed)
-        hipLaunchKernelGGL(k_permute<float>, ga, blk, 0, s, d);
+        hipLaunchKernelGGL(k_permute<float>, gall, blk, 0, s, d);
     else
-        hipLaunchKernelGGL(k_permute<double>, ga, blk, 0, s, d);
-    // one workgroup per possible item: the search needs the registers
-    const dim3 gall(xcd_grid(g.nblk));
+        hipLaunchKernelGGL(k_permute<double>, gall, blk, 0, s, d);
+    // the search too (it needs the registers a loop would keep)
     if (g.nbr_cu == 4)
         hipLaunchKernelGGL(k_nbr<4>, gall, blk, 0, s, d);
     else

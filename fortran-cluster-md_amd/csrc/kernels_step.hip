@@ -31,7 +31,7 @@ __device__ void set_cell_grid(const TrajParams &tp, TrajScalars &ts)
         ts.cside[k] = tp.box[k] / (double)n[k];
         ts.icside[k] = 1.0 / ts.cside[k];
     }
-    if ((long long)n[0] * n[1] * n[2] > (long long)tp.ncpad) {
+    if ((long long)n[0] * n[1] * n[2] >= (long long)tp.ncpad) {  // one cell past the grid is kept
         ts.err = ERR_CELLS;
         return;
     }
