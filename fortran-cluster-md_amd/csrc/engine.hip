@@ -327,6 +327,9 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     s->g.nblk = (int)(abase / BLK);
     s->g.ncblk = (int)(cbase / SCAN_CHUNK);
     s->g.ntraj = ntraj;
+    s->g.list16 = 1;  // 16-bit list entries when every trajectory's slots fit
+    for (auto &t : s->tp)
+        if (t.npad > LIST16_MAX) s->g.list16 = 0;
     s->g.force_u = 4;
     s->g.nbr_cu = 2;
     s->g.wl_grid = 2048;
@@ -355,7 +358,12 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.wl_size, (long long)WL_LISTS * ntraj);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, s->g.ncblk);
-    A(d.nn, N) A(d.list, lbase) A(d.posf, N);
+    A(d.nn, N) A(d.posf, N);
+    if (s->g.list16) {
+        A(d.list16, lbase);
+    } else {
+        A(d.list32, lbase);
+    }
     A(d.edeg, N) A(d.edeg1, N) A(d.edges, N * EMAX) A(d.parent, N) A(d.parentf, N) A(d.liq, N);
     A(d.cs_tmp, N) A(d.cgs_tmp, N) A(d.csf_tmp, N) A(d.cek_tmp, N) A(d.cekf_tmp, N);
     A(d.liqo, N) A(d.rank, N) A(d.o_cl, N) A(d.o_cs, N) A(d.o_cgs, N) A(d.o_clf, N) A(d.o_ccf, N);
@@ -662,14 +670,20 @@ int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *at
     HIPCHK(hipMemcpy(oid.data(), s->d.oid + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
     const long long lo = t.lbase, nslots = (long long)t.npad * t.lwidth;
     std::vector<int> lst(nslots);
-    HIPCHK(hipMemcpy(lst.data(), s->d.list + lo, sizeof(int) * nslots, hipMemcpyDeviceToHost));
+    if (s->g.list16) {
+        std::vector<unsigned short> l16(nslots);
+        HIPCHK(hipMemcpy(l16.data(), s->d.list16 + lo, sizeof(unsigned short) * nslots, hipMemcpyDeviceToHost));
+        for (long long k = 0; k < nslots; ++k) lst[k] = l16[k];
+    } else {
+        HIPCHK(hipMemcpy(lst.data(), s->d.list32 + lo, sizeof(int) * nslots, hipMemcpyDeviceToHost));
+    }
     // neighbours (reference numbers, j > i) of each reference atom i
     std::vector<std::vector<int>> nb(n);
     for (int r = 0; r < n; ++r) {
         const int oi = oid[r];
         const long long base = list_slot(t, r) - lo;
         for (int q = 0; q < nn[r]; ++q) {
-            const int j = lst[base + (long long)q * TILE] - t.abase;
+            const int j = lst[base + (long long)q * TILE];  // trajectory-relative
             const int oj = oid[j];
             if (oj > oi) nb[oi - 1].push_back(oj);
         }

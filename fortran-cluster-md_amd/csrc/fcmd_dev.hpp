@@ -83,6 +83,10 @@ __host__ __device__ inline long long list_slot(const TrajParams &tp, int rel)
     return tp.lbase + (long long)(rel / TILE) * tp.lwidth * TILE + (rel & (TILE - 1));
 }
 
+// the list array of entry type LT
+template <typename LT>
+__host__ __device__ inline LT *list_base(const struct DevPtrs &d);
+
 // per-trajectory results of one cluster analysis
 struct ClusterScalars {
     double cekf0;                    // cekf(0): KE of dense but not liquid-like A atoms
@@ -135,9 +139,13 @@ struct DevPtrs {
     int *cell_cnt, *cell_start, *sorted;
     int *atom_cell, *atom_slot;      // per slot at binning: packed cell (CELL_BITS each), slot in cell
     long long *scan_tmp_ll;          // chunk totals for segmented scans
-    // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE]
+    // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE],
+    // each the neighbour's trajectory-relative slot: 16-bit (list16) when
+    // every trajectory has at most 65536 slots (half the list traffic), else
+    // 32-bit (list32); the other pointer is null
     int *nn;
-    int *list;
+    unsigned short *list16;
+    int *list32;
     // per slot: the FP32 offset of the atom from the centre of its cell at the
     // last list build, and that cell packed in w (make_posf).  Written at each
     // list build (the list search reads it); on the mixed path also kept
@@ -159,5 +167,19 @@ struct DevPtrs {
     int *o_hacc;                     // cluster-size counts over analyses (hbase + size)
     int *hacc_stop;                  // per trajectory: counting has stopped
 };
+
+template <>
+__host__ __device__ inline unsigned short *list_base<unsigned short>(const DevPtrs &d)
+{
+    return d.list16;
+}
+template <>
+__host__ __device__ inline int *list_base<int>(const DevPtrs &d)
+{
+    return d.list32;
+}
+
+// the largest trajectory (in slots) whose list entries fit 16 bits
+constexpr long long LIST16_MAX = 65536;
 
 }  // namespace fcmd

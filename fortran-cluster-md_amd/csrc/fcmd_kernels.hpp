@@ -14,6 +14,7 @@ struct LaunchDims {
     int ncblk;   // cell chunks
     int nhblk;   // histogram chunks
     int ntraj;
+    int list16;  // list entries are 16-bit (DevPtrs::list16)
     // tunables (none changes a result bit; fcmd_autotune picks them):
     int force_u;  // neighbours per load batch in the force kernel: 2, 4 or 8
     int nbr_cu;   // candidates per load batch in the list search: 2 or 4

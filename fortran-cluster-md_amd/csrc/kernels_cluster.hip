@@ -100,7 +100,8 @@ __global__ __launch_bounds__(BLK) void k_cl_init(DevPtrs d)
     }
 }
 
-// edges within atskin; degrees; liquid-like flag (tWF)
+// edges within atskin; degrees; liquid-like flag (tWF).  LT: list entry type.
+template <typename LT>
 __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
 {
     const int b = xcd_block(blockIdx.x, d.nblk);
@@ -119,14 +120,14 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
     const double bf1 = fint(ts.atrskin / tp.atr) + 2.0;
     const double bf2 = bf1 + 0.5;
     const double rijc2 = tp.atskin * tp.atskin;
-    const int *lp = d.list + list_slot(tp, rel);
+    const LT *lp = list_base<LT>(d) + list_slot(tp, rel);
     const int nn = d.nn[i];
     int deg = 0, deg1 = 0;
     int *ed = d.edges + (size_t)i * EMAX;
     const bool only_a = tp.stoddard == ST_NM;
     if (!(only_a && ti != 0)) {
         for (int s = 0; s < nn; ++s) {
-            const int j = lp[(size_t)s * TILE];
+            const int j = tp.abase + (int)lp[(size_t)s * TILE];
             const int tj = d.typ[j];
             if (only_a && tj != 0) continue;
             double dx = xi - d.x[j];
@@ -362,7 +363,10 @@ void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int war
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cl_init, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_edges, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    if (g.list16)
+        hipLaunchKernelGGL(k_cl_edges<unsigned short>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    else
+        hipLaunchKernelGGL(k_cl_edges<int>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     hipLaunchKernelGGL(k_cl_hook, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     hipLaunchKernelGGL(k_cl_count, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     if (fre) hipLaunchKernelGGL(k_cl_rank, dim3(g.ntraj), dim3(BLK), 0, s, d);

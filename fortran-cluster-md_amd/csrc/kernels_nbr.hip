@@ -441,26 +441,32 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
         int zz = cz + oz;
         zz += (zz < 0) ? ncz : (zz >= ncz ? -ncz : 0);
         // per row q: first range [A[q], A[q] + la[q]), then the rest from
-        // B[q] + la[q] (absolute slots); e0, e1, n: the plane's row ends
+        // B[q] + la[q] (absolute slots); e0, e1, n: the plane's row ends.
+        // All twelve bounds are loaded before any is used (a missing row or
+        // second range reads a harmless in-range start and gets length 0),
+        // so the plane costs one round trip.
+        const bool wrap = xb1 >= xb0;
+        const int xs0 = wrap ? xb0 : 0, xs1 = wrap ? xb1 : -1;
+        int sa[3], ea[3], sb[3], eb[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int oy = ly + q <= hy ? ly + q : 0;
+            int yy = cy + oy;
+            yy += (yy < 0) ? ncy : (yy >= ncy ? -ncy : 0);
+            const int row = (zz * ncy + yy) * ncx;
+            sa[q] = cstart[row + xa0];
+            ea[q] = cstart[row + xa1 + 1];
+            sb[q] = cstart[row + xs0];
+            eb[q] = cstart[row + xs1 + 1];
+        }
         int A[3], la[3], B[3], len[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const int oy = ly + q;
-            A[q] = la[q] = B[q] = len[q] = 0;
-            if (oy <= hy) {
-                int yy = cy + oy;
-                yy += (yy < 0) ? ncy : (yy >= ncy ? -ncy : 0);
-                const int row = (zz * ncy + yy) * ncx;
-                const int a0 = cstart[row + xa0];
-                la[q] = cstart[row + xa1 + 1] - a0;
-                len[q] = la[q];
-                A[q] = tp.abase + a0;
-                if (xb1 >= xb0) {
-                    const int b0 = cstart[row + xb0];
-                    len[q] += cstart[row + xb1 + 1] - b0;
-                    B[q] = tp.abase + b0 - la[q];
-                }
-            }
+            const bool ok = ly + q <= hy;
+            la[q] = ok ? ea[q] - sa[q] : 0;
+            len[q] = la[q] + (ok && wrap ? eb[q] - sb[q] : 0);
+            A[q] = tp.abase + sa[q];
+            B[q] = tp.abase + sb[q] - la[q];
         }
         // the plane's three rows walked as one sequence (lanes whose rows
         // differ in length stay busy across row ends)
@@ -509,7 +515,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
 // neighbours stops the trajectory with LISTFULL (the reference stops when its
 // list array is nearly full, neighbor-pbc.f:89-93); its count is clamped so
 // that no later kernel reads past its slots.
-template <int CU>
+template <int CU, typename LT>
 __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
 {
     __shared__ int red[2];
@@ -525,9 +531,9 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         const int width = tp.lwidth;
         int cnt = 0, half = 0;
         if (rel < tp.natom) {
-            int *out = d.list + list_slot(tp, rel);
+            LT *out = list_base<LT>(d) + list_slot(tp, rel);
             cnt = candidates<CU>(d, tp, ts, i, half, [&](int j, int k) {
-                if (k < width) __builtin_nontemporal_store(j, out + (size_t)k * TILE);
+                if (k < width) __builtin_nontemporal_store((LT)(j - tp.abase), out + (size_t)k * TILE);
             });
         }
         if (cnt > width) {
@@ -592,10 +598,17 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     else
         hipLaunchKernelGGL(k_permute<double>, gall, blk, 0, s, d);
     // the search too (it needs the registers a loop would keep)
-    if (g.nbr_cu == 4)
-        hipLaunchKernelGGL(k_nbr<4>, gall, blk, 0, s, d);
-    else
-        hipLaunchKernelGGL(k_nbr<2>, gall, blk, 0, s, d);
+    if (g.list16) {
+        if (g.nbr_cu == 4)
+            hipLaunchKernelGGL((k_nbr<4, unsigned short>), gall, blk, 0, s, d);
+        else
+            hipLaunchKernelGGL((k_nbr<2, unsigned short>), gall, blk, 0, s, d);
+    } else {
+        if (g.nbr_cu == 4)
+            hipLaunchKernelGGL((k_nbr<4, int>), gall, blk, 0, s, d);
+        else
+            hipLaunchKernelGGL((k_nbr<2, int>), gall, blk, 0, s, d);
+    }
 }
 
 }  // namespace fcmd

@@ -359,27 +359,97 @@ constexpr int STAGE = 16;
 // step: nontemporal accesses keep it from displacing the neighbours'
 // records, which the same and nearby wavefronts gather again from L2.
 
-// An atom's list index at slot s: from the LDS stage for its first STAGE slots
-// (all of them loaded at once before the pair loop, so the loop waits only on
-// neighbour gathers), from HBM beyond.
-__device__ __forceinline__ int list_at(const int (*stage)[BLK], const int *lp, int s)
+// list slots loaded together with the count, before it is known
+constexpr int SU = 8;
+
+// An atom's list (entries of type LT, trajectory-relative): its count, its
+// first slot and its first SU entries, all loaded at once (every atom owns
+// lwidth slots, whatever its count), before anything waits on memory.
+// Entries travel as the loaded dwords ("words") and are decoded only once
+// every load of a batch has been issued: a load under a per-lane condition
+// whose result were used at once would make the compiler wait for it inside
+// the condition, one load at a time.
+template <typename LT>
+struct ListHead {
+    int nn, base;
+    const unsigned *lp;  // the dword holding the atom's entry of list position 0
+    int sh;              // the entry's bit offset in its dword (16-bit entries)
+    unsigned fill;       // a word that decodes to the atom itself
+    unsigned head[SU];   // words
+    __device__ __forceinline__ ListHead(const DevPtrs &d, const TrajParams &tp, int i, int rel)
+    {
+        nn = __builtin_nontemporal_load(d.nn + i);
+        base = tp.abase;
+        const LT *p = list_base<LT>(d) + list_slot(tp, rel);
+        // 16-bit entries are read as the aligned dword that holds them (an
+        // atom pair shares it): whole-dword loads, not sub-dword ones, which
+        // are slow here.  Slots are even-aligned: lbase and the tile offsets
+        // are even, and rel & 1 is the lane's parity.
+        sh = sizeof(LT) == 2 ? 16 * (rel & 1) : 0;
+        lp = (const unsigned *)(sizeof(LT) == 2 ? p - (rel & 1) : p);
+        fill = (unsigned)rel << sh;
+#pragma unroll
+        for (int s = 0; s < SU; ++s) head[s] = s < tp.lwidth ? word(s) : fill;
+    }
+    // the word of list position s, from HBM
+    __device__ __forceinline__ unsigned word(int s) const
+    {
+        constexpr int STRIDE = TILE * (int)sizeof(LT) / 4;  // dwords per list position
+        return __builtin_nontemporal_load(lp + (size_t)s * STRIDE);
+    }
+    // the neighbour slot a word holds
+    __device__ __forceinline__ int decode(unsigned w) const
+    {
+        return base + (int)(sizeof(LT) == 2 ? (w >> sh) & 0xffffu : w);
+    }
+    // the neighbour slots of positions s0 .. s0+U-1 (the atom itself past
+    // nn); the first STAGE positions come from the LDS stage (all loaded
+    // before the pair loop, so the loop waits only on neighbour gathers)
+    template <int U>
+    __device__ __forceinline__ void batch(const unsigned (*stage)[BLK], int s0, int (&jj)[U]) const
+    {
+        unsigned w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = s0 + u;
+            w[u] = s < nn ? (s < STAGE ? stage[s][threadIdx.x] : word(s)) : fill;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = decode(w[u]);
+    }
+};
+
+// Walks an atom's nn list slots in batches of U (U <= SU): load(s0) issues a
+// batch's gathers, sum(s0) consumes them.  With the head slots loaded with
+// the count, the first batch's gathers wait on one round trip instead of
+// two; the other STAGE - SU staged slots are loaded meanwhile and written to
+// LDS after the first batch is in flight.  The stage holds words.
+template <int U, typename H, typename Load, typename Sum>
+__device__ __forceinline__ void walk_list(unsigned (*stage)[BLK], const H &h, Load load, Sum sum)
 {
-    return s < STAGE ? stage[s][threadIdx.x] : __builtin_nontemporal_load(lp + (size_t)s * TILE);
+    static_assert(U <= SU && SU <= STAGE, "batch wider than the early slots");
+    const int nn = h.nn;
+#pragma unroll
+    for (int s = 0; s < SU; ++s) stage[s][threadIdx.x] = s < nn ? h.head[s] : h.fill;
+    unsigned tail[STAGE - SU];
+#pragma unroll
+    for (int s = SU; s < STAGE; ++s) tail[s - SU] = s < nn ? h.word(s) : h.fill;
+    if (nn > 0) {
+        load(0);
+#pragma unroll
+        for (int s = SU; s < STAGE; ++s) stage[s][threadIdx.x] = tail[s - SU];
+        sum(0);
+    }
+    for (int s0 = U; s0 < nn; s0 += U) {
+        load(s0);
+        sum(s0);
+    }
 }
 
-__device__ __forceinline__ void stage_list(int (*stage)[BLK], const int *lp, int nn, int self)
-{
-    int tmp[STAGE];
-#pragma unroll
-    for (int s = 0; s < STAGE; ++s) tmp[s] = s < nn ? __builtin_nontemporal_load(lp + (size_t)s * TILE) : self;
-#pragma unroll
-    for (int s = 0; s < STAGE; ++s) stage[s][threadIdx.x] = tmp[s];
-}
-
-template <int FORM, int U>
+template <int FORM, int U, typename H>
 __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
-                                              int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                              double &e, int (*stage)[BLK])
+                                              int i, const H &h, int ti, double &fx, double &fy, double &fz,
+                                              double &e, unsigned (*stage)[BLK])
 {
     const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
@@ -387,15 +457,12 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
     const double bf1 = 2.0 + fint(ts.atrskin / tp.atr);
     const double bf2 = bf1 + 0.5;
     const double rc2 = tp.atrcut * tp.atrcut;
-    const int nn = __builtin_nontemporal_load(d.nn + i);
-    const int *lp = d.list + list_slot(tp, rel);
-    stage_list(stage, lp, nn, i);
+    const int nn = h.nn;
     fx = fy = fz = 0.0;
-    for (int s0 = 0; s0 < nn; s0 += U) {
-        int jj[U], tj[U];
-        double xj[U], yj[U], zj[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? list_at(stage, lp, s0 + u) : i;
+    int jj[U], tj[U];
+    double xj[U], yj[U], zj[U];
+    auto load = [&](int s0) {
+        h.batch(stage, s0, jj);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             xj[u] = d.x[jj[u]];
@@ -403,6 +470,8 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
             zj[u] = d.z[jj[u]];
             tj[u] = (FORM == ACC_1) ? 0 : d.typ[jj[u]];
         }
+    };
+    auto sum = [&](int s0) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (s0 + u >= nn) break;
@@ -430,7 +499,8 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
                 fz = fz + fij * dz;
             }
         }
-    }
+    };
+    walk_list<U>(stage, h, load, sum);
 }
 
 // one coordinate of a mixed-path separation: cell difference wrapped to a
@@ -454,12 +524,11 @@ __device__ __forceinline__ float sep_mixed(int ci, int cj, int nc, float side, f
 // compensation, energies summed in FP64.  SMALL: a separate loop for boxes
 // with fewer than 3 cells on some side, so the usual case carries no rounding
 // image.
-template <int FORM, bool SMALL, int U>
+template <int FORM, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
-                                               int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                               double &e, int (*stage)[BLK])
+                                               int i, const float4 &pi, const H &h, int ti, double &fx,
+                                               double &fy, double &fz, double &e, unsigned (*stage)[BLK])
 {
-    const float4 pi = d.posf[i];
     int cxi, cyi, czi;
     unpack_cell(pi.w, cxi, cyi, czi);
     const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
@@ -467,20 +536,19 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
     const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
     const float rc2f = (float)(tp.atrcut * tp.atrcut);
-    const int nn = __builtin_nontemporal_load(d.nn + i);
-    const int *lp = d.list + list_slot(tp, rel);
-    stage_list(stage, lp, nn, i);
+    const int nn = h.nn;
     KahanF kx, ky, kz;
-    for (int s0 = 0; s0 < nn; s0 += U) {
-        int jj[U], tj[U];
-        float4 pj[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (s0 + u < nn) ? list_at(stage, lp, s0 + u) : i;
+    int jj[U], tj[U];
+    float4 pj[U];
+    auto load = [&](int s0) {
+        h.batch(stage, s0, jj);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             pj[u] = d.posf[jj[u]];
             tj[u] = (FORM == ACC_1) ? 0 : d.typ[jj[u]];
         }
+    };
+    auto sum = [&](int s0) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (s0 + u >= nn) break;
@@ -507,36 +575,44 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
                 kz.add(fij * dzf);
             }
         }
-    }
+    };
+    walk_list<U>(stage, h, load, sum);
     fx = (double)kx.s;
     fy = (double)ky.s;
     fz = (double)kz.s;
 }
 
-template <typename AccT, int FORM, int U>
+template <typename AccT, int FORM, int U, typename LT>
 __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                          int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                         double &e, int (*stage)[BLK])
+                                         double &e, unsigned (*stage)[BLK])
 {
+    // the list's first loads go out before anything waits on memory
+    const ListHead<LT> h(d, tp, i, rel);
     if constexpr (sizeof(AccT) == 4) {
+        const float4 pi = d.posf[i];
         if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
+            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, stage);
         else
-            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
+            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, stage);
     } else
-        pair_sum_fp64<FORM, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
+        pair_sum_fp64<FORM, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, stage);
 }
 
 // LJ forces over the full neighbour list plus the integrator's fused
 // epilogue.  MULTI = false: every trajectory of the launch is single-species
 // (ACC_1); MULTI = true: each trajectory uses its own form.  U: neighbours
 // per load batch (a tunable; the pairs are summed in list order whatever U).
-template <typename AccT, bool MULTI, int U>
-__global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
+// LT: list entry type.  Single-species kernels of up to 4 neighbours a
+// batch are held to 72 VGPRs (7 wavefronts a SIMD; without spills), which
+// the 16-bit list decode would otherwise just exceed.
+template <typename AccT, bool MULTI, int U, typename LT>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U <= 4 ? 7 : 1))) void k_force(
+    DevPtrs d, int mode)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
     __shared__ double lds[4];
-    __shared__ int stage[STAGE][BLK];  // each thread's first list slots
+    __shared__ unsigned stage[STAGE][BLK];  // each thread's first list slots
     const int b = xcd_block(blockIdx.x, d.nblk);
     if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
@@ -550,11 +626,19 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
     double e = 0.0, eks = 0.0, ekfree = 0.0;
     if (act) {
         const int ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
+        // the velocities, loaded before the pair sum so that they arrive
+        // while it runs
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        if (mode == FORCE_STEP) {
+            vx = __builtin_nontemporal_load(d.vx + i);
+            vy = __builtin_nontemporal_load(d.vy + i);
+            vz = __builtin_nontemporal_load(d.vz + i);
+        }
         double fx, fy, fz;
         if (form1)
-            pair_sum<AccT, ACC_1, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
+            pair_sum<AccT, ACC_1, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
         else
-            pair_sum<AccT, ACC_M, U>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
+            pair_sum<AccT, ACC_M, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
         // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:408-414
         double ax, ay, az;
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
@@ -586,8 +670,6 @@ __global__ __launch_bounds__(BLK) void k_force(DevPtrs d, int mode)
         }
         if (mode == FORCE_STEP) {
             const double dt = tp.dt;
-            double vx = __builtin_nontemporal_load(d.vx + i), vy = __builtin_nontemporal_load(d.vy + i),
-                   vz = __builtin_nontemporal_load(d.vz + i);
             if (tp.integrator == INT_NH) {
                 const double dth = dt / 2;
                 const double wx = vx + dth * ax, wy = vy + dth * ay, wz = vz + dth * az;
@@ -702,16 +784,25 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
     launch_traj_pre(d, g, s);
 }
 
-template <typename AccT, bool MULTI>
-static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+template <typename AccT, bool MULTI, typename LT>
+static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
 {
     const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
     if (g.force_u == 2)
-        hipLaunchKernelGGL((k_force<AccT, MULTI, 2>), grid, blk, 0, s, d, mode);
+        hipLaunchKernelGGL((k_force<AccT, MULTI, 2, LT>), grid, blk, 0, s, d, mode);
     else if (g.force_u == 8)
-        hipLaunchKernelGGL((k_force<AccT, MULTI, 8>), grid, blk, 0, s, d, mode);
+        hipLaunchKernelGGL((k_force<AccT, MULTI, 8, LT>), grid, blk, 0, s, d, mode);
     else
-        hipLaunchKernelGGL((k_force<AccT, MULTI, 4>), grid, blk, 0, s, d, mode);
+        hipLaunchKernelGGL((k_force<AccT, MULTI, 4, LT>), grid, blk, 0, s, d, mode);
+}
+
+template <typename AccT, bool MULTI>
+static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+{
+    if (g.list16)
+        launch_force_lt<AccT, MULTI, unsigned short>(d, g, mode, s);
+    else
+        launch_force_lt<AccT, MULTI, int>(d, g, mode, s);
 }
 
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
