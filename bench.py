@@ -15,7 +15,9 @@ Reported:
   speedup          value / fp64_plain
   accuracy         RMS relative force error of the mixed path vs FP64 on the
                    same state; energy-drift ratio mixed/FP64
-  roofline         dominant kernel, HIP events on the engine stream
+  roofline         dominant kernel: its mean duration over K more steps, run
+                   without the graph, from HIP events each launch records at
+                   the kernel's start and end on the engine stream
   cpu_baseline     the reference Fortran (flang -O3) on the host cores
   dtype            f32+f64: pair arithmetic in FP32, state, integrator and
                    reductions in FP64 (the fp64_* fields are FP64 throughout)
@@ -225,8 +227,8 @@ def main():
         log(f"[bench] {name}: {K} steps in {wall:.3f} s (events {ev_ms / 1e3:.3f} s) -> "
             f"{res[name]['rate']:.4e} atom-steps/s")
         if name == "mixed":
-            log("[bench] mixed: kernel timing (HIP events)")
-            kt = s.time_kernels(20)
+            log(f"[bench] mixed: kernel timing (HIP events, {K} more steps)")
+            kt = s.time_kernels(K)
             res[name]["kernels"] = kt
             res[name]["nlist_k"] = s.list_entries()
             # the drop-in hands host buffers over the boundary: one download

@@ -907,38 +907,42 @@ int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
 {
     if (!s || nsteps <= 0 || !avg_ms) return FCMD_ERR_ARG;
     const int nk = FCMD_NTIMED;  // k_drift, k_traj_pre, rebuild sequence, k_force, k_traj_post, kick
-    std::vector<hipEvent_t> ev((size_t)nsteps * (nk + 1));
+    // per step, the start and end of each of the first nk - 1 groups,
+    // recorded by the launches themselves (no event packets between kernels,
+    // so the step runs as in a run); the kick once at the end, bracketed
+    const int ne = 2 * (nk - 1);
+    std::vector<hipEvent_t> ev((size_t)nsteps * ne + 2);
     // timing-only events: no system-scope fence (cache writeback) at each
-    // record, which would otherwise be charged to the bracketed kernel
+    // record, which would otherwise be charged to the timed kernel
     for (auto &e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     for (int k = 0; k < nsteps; ++k) {
-        hipEvent_t *e = ev.data() + (size_t)k * (nk + 1);
-        HIPCHK(hipEventRecord(e[0], s->stream));
-        launch_drift_atoms(s->d, s->g, s->mixed, s->stream);
-        HIPCHK(hipEventRecord(e[1], s->stream));
-        launch_traj_pre(s->d, s->g, s->stream);
-        HIPCHK(hipEventRecord(e[2], s->stream));
-        launch_rebuild(s->d, s->g, s->mixed, s->stream);
-        HIPCHK(hipEventRecord(e[3], s->stream));
-        launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
-        HIPCHK(hipEventRecord(e[4], s->stream));
-        launch_traj_post(s->d, s->g, FORCE_STEP, s->stream);
-        HIPCHK(hipEventRecord(e[5], s->stream));
-        // the run loop defers the NH update to the next drift; only the last
-        // step of a run (or an analysis) pays it, so it is timed once
-        if (k == nsteps - 1) finalize(s);
-        HIPCHK(hipEventRecord(e[6], s->stream));
+        hipEvent_t *e = ev.data() + (size_t)k * ne;
+        const KTimer kt[5] = {{e[0], e[1]}, {e[2], e[3]}, {e[4], e[5]}, {e[6], e[7]}, {e[8], e[9]}};
+        launch_drift_atoms(s->d, s->g, s->mixed, s->stream, kt[0]);
+        launch_traj_pre(s->d, s->g, s->stream, kt[1]);
+        launch_rebuild(s->d, s->g, s->mixed, s->stream, kt[2]);
+        launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, kt[3]);
+        launch_traj_post(s->d, s->g, FORCE_STEP, s->stream, kt[4]);
     }
+    // the run loop defers the NH update to the next drift; only the last step
+    // of a run (or an analysis) pays it, so it is timed once
+    hipEvent_t *ek = ev.data() + (size_t)nsteps * ne;
+    HIPCHK(hipEventRecord(ek[0], s->stream));
+    finalize(s);
+    HIPCHK(hipEventRecord(ek[1], s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int c = 0; c < nk; ++c) avg_ms[c] = 0.0;
     for (int k = 0; k < nsteps; ++k) {
-        hipEvent_t *e = ev.data() + (size_t)k * (nk + 1);
-        for (int c = 0; c < nk; ++c) {
+        hipEvent_t *e = ev.data() + (size_t)k * ne;
+        for (int c = 0; c < nk - 1; ++c) {
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e[c], e[c + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, e[2 * c], e[2 * c + 1]));
             avg_ms[c] += ms / nsteps;
         }
     }
+    float mk = 0.f;
+    HIPCHK(hipEventElapsedTime(&mk, ek[0], ek[1]));
+    avg_ms[nk - 1] = mk / nsteps;
     for (auto &e : ev) (void)hipEventDestroy(e);
     return sync_err(s);
 }

@@ -1,6 +1,8 @@
 // fcmd_kernels.hpp -- host-side launchers of the device kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "fcmd_dev.hpp"
 
 namespace fcmd {
@@ -21,19 +23,36 @@ struct LaunchDims {
     int wl_grid;  // workgroups of each rebuild kernel: 1024, 2048 or 4096
 };
 
+// Events recorded at a kernel's start and end (hipExtLaunchKernelGGL), for
+// timing it alone; a launch sequence records t0 at its first kernel's start
+// and t1 at its last one's end.  None: ordinary launches.
+struct KTimer {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+// a kernel launch that records kt's events, if any
+template <typename F, typename... Args>
+inline void launch_timed(F kern, dim3 grid, dim3 blk, hipStream_t s, const KTimer &kt, Args... args)
+{
+    if (kt.t0 || kt.t1)
+        hipExtLaunchKernelGGL(kern, grid, blk, 0, s, kt.t0, kt.t1, 0, args...);
+    else
+        hipLaunchKernelGGL(kern, grid, blk, 0, s, args...);
+}
+
 // launch_drift = launch_drift_atoms (k_drift) + launch_traj_pre (k_traj_pre);
 // launch_force = launch_force_atoms (k_force) + launch_traj_post (k_traj_post)
 void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
-void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
-void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s);
+void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt = {});
+void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s, const KTimer &kt = {});
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
                   hipStream_t s);
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                        hipStream_t s);
-void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s);
+                        hipStream_t s, const KTimer &kt = {});
+void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt = {});
 // applies any pending Nose-Hoover final update and clears the flags
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
-void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
+void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt = {});
 // after positions of trajectory t are replaced from the host (mixed path)
 void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s);
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s);

@@ -578,13 +578,14 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
 }
 
 // ---------------------------------------------------------------------------
-void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
+void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt)
 {
+    const KTimer first{kt.t0, nullptr}, last{nullptr, kt.t1};
     const dim3 blk(BLK);
     const int wg = g.wl_grid;  // workgroups per rebuild kernel (a tunable)
     const dim3 gc(xcd_grid(std::min(g.ncblk, wg)));
     const dim3 gj(xcd_grid(std::min(g.ntraj, wg)));
-    hipLaunchKernelGGL(k_wl_fill, dim3(g.ntraj), blk, 0, s, d, g.ntraj);
+    launch_timed(k_wl_fill, dim3(g.ntraj), blk, s, first, d, g.ntraj);
     // one workgroup per possible atom block where a block's work is a short
     // chain of dependent accesses (no loop: every block starts at once)
     const dim3 gall(xcd_grid(g.nblk));
@@ -600,14 +601,14 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     // the search too (it needs the registers a loop would keep)
     if (g.list16) {
         if (g.nbr_cu == 4)
-            hipLaunchKernelGGL((k_nbr<4, unsigned short>), gall, blk, 0, s, d);
+            launch_timed(k_nbr<4, unsigned short>, gall, blk, s, last, d);
         else
-            hipLaunchKernelGGL((k_nbr<2, unsigned short>), gall, blk, 0, s, d);
+            launch_timed(k_nbr<2, unsigned short>, gall, blk, s, last, d);
     } else {
         if (g.nbr_cu == 4)
-            hipLaunchKernelGGL((k_nbr<4, int>), gall, blk, 0, s, d);
+            launch_timed(k_nbr<4, int>, gall, blk, s, last, d);
         else
-            hipLaunchKernelGGL((k_nbr<2, int>), gall, blk, 0, s, d);
+            launch_timed(k_nbr<2, int>, gall, blk, s, last, d);
     }
 }
 

@@ -765,17 +765,17 @@ void launch_neighbor_entry(const DevPtrs &d, int traj, int ntraj, int step, int 
     hipLaunchKernelGGL(k_neighbor_entry, dim3(1), dim3(64), 0, s, d, traj, ntraj, step, step_out);
 }
 
-void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
+void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt)
 {
     if (mixed)
-        hipLaunchKernelGGL(k_drift<float>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+        launch_timed(k_drift<float>, dim3(xcd_grid(g.nblk)), dim3(BLK), s, kt, d);
     else
-        hipLaunchKernelGGL(k_drift<double>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+        launch_timed(k_drift<double>, dim3(xcd_grid(g.nblk)), dim3(BLK), s, kt, d);
 }
 
-void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s)
+void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s, const KTimer &kt)
 {
-    hipLaunchKernelGGL(k_traj_pre, dim3(g.ntraj), dim3(BLK), 0, s, d);
+    launch_timed(k_traj_pre, dim3(g.ntraj), dim3(BLK), s, kt, d);
 }
 
 void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
@@ -785,45 +785,45 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
 }
 
 template <typename AccT, bool MULTI, typename LT>
-static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt)
 {
     const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
     if (g.force_u == 2)
-        hipLaunchKernelGGL((k_force<AccT, MULTI, 2, LT>), grid, blk, 0, s, d, mode);
+        launch_timed(k_force<AccT, MULTI, 2, LT>, grid, blk, s, kt, d, mode);
     else if (g.force_u == 8)
-        hipLaunchKernelGGL((k_force<AccT, MULTI, 8, LT>), grid, blk, 0, s, d, mode);
+        launch_timed(k_force<AccT, MULTI, 8, LT>, grid, blk, s, kt, d, mode);
     else
-        hipLaunchKernelGGL((k_force<AccT, MULTI, 4, LT>), grid, blk, 0, s, d, mode);
+        launch_timed(k_force<AccT, MULTI, 4, LT>, grid, blk, s, kt, d, mode);
 }
 
 template <typename AccT, bool MULTI>
-static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt)
 {
     if (g.list16)
-        launch_force_lt<AccT, MULTI, unsigned short>(d, g, mode, s);
+        launch_force_lt<AccT, MULTI, unsigned short>(d, g, mode, s, kt);
     else
-        launch_force_lt<AccT, MULTI, int>(d, g, mode, s);
+        launch_force_lt<AccT, MULTI, int>(d, g, mode, s, kt);
 }
 
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                        hipStream_t s)
+                        hipStream_t s, const KTimer &kt)
 {
     if (mixed) {
         if (single_species)
-            launch_force_u<float, false>(d, g, mode, s);
+            launch_force_u<float, false>(d, g, mode, s, kt);
         else
-            launch_force_u<float, true>(d, g, mode, s);
+            launch_force_u<float, true>(d, g, mode, s, kt);
     } else {
         if (single_species)
-            launch_force_u<double, false>(d, g, mode, s);
+            launch_force_u<double, false>(d, g, mode, s, kt);
         else
-            launch_force_u<double, true>(d, g, mode, s);
+            launch_force_u<double, true>(d, g, mode, s, kt);
     }
 }
 
-void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s)
+void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt)
 {
-    hipLaunchKernelGGL(k_traj_post, dim3(g.ntraj), dim3(BLK), 0, s, d, mode);
+    launch_timed(k_traj_post, dim3(g.ntraj), dim3(BLK), s, kt, d, mode);
 }
 
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,

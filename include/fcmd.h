@@ -176,11 +176,14 @@ int fcmd_get_size_counts(fcmd_system *s, int traj, int *counts, int *stopped);
 /* Time of the last fcmd_run in milliseconds (HIP events on the run stream). */
 int fcmd_last_run_ms(fcmd_system *s, double *total_ms);
 
-/* Run nsteps steps (no graph) with HIP events on the engine stream around
- * each launch; avg_ms[FCMD_NTIMED] receives the mean milliseconds per step of
+/* Run nsteps steps (no graph) and time them with HIP events on the engine
+ * stream; avg_ms[FCMD_NTIMED] receives the mean milliseconds per step of
  * {k_drift, k_traj_pre, list-rebuild sequence, k_force, k_traj_post,
- * Nose-Hoover kick}.  The kick is deferred to the next drift inside a run, so
- * it is timed once over the nsteps and its entry is that time / nsteps. */
+ * Nose-Hoover kick}.  The events are recorded by the launches themselves,
+ * at the start and end of each kernel (of the rebuild sequence: its first
+ * kernel's start, its last one's end), so the steps run as in a run.  The
+ * kick is deferred to the next drift inside a run, so it is timed once after
+ * the nsteps and its entry is that time / nsteps. */
 #define FCMD_NTIMED 6
 int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms);
 
