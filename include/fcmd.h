@@ -91,7 +91,9 @@ const char *fcmd_error_text(int code);
  * may have; 0 selects 128.  It takes the role of const.inc mxnlist: a list
  * rebuild that finds an atom with more neighbours fails with
  * FCMD_ERR_LISTFULL, as the reference stops when its list array is nearly
- * full (neighbor-pbc.f:89-93).  Memory: 4 * list_width bytes per atom. */
+ * full (neighbor-pbc.f:89-93).  Memory: 2 * list_width bytes per atom when
+ * every trajectory has at most 65536 atom slots (natom rounded up to 256),
+ * else 4 * list_width. */
 int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int precision,
                 int list_width);
 void fcmd_destroy(fcmd_system *s);
