@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -327,9 +328,13 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     s->g.nblk = (int)(abase / BLK);
     s->g.ncblk = (int)(cbase / SCAN_CHUNK);
     s->g.ntraj = ntraj;
-    s->g.list16 = 1;  // 16-bit list entries when every trajectory's slots fit
+    // 16-bit list entries when every trajectory's slots fit, unless
+    // FCMD_LIST_BITS=32 asks for 32-bit ones (same lists, same results)
+    s->g.list16 = 1;
     for (auto &t : s->tp)
         if (t.npad > LIST16_MAX) s->g.list16 = 0;
+    if (const char *lb = std::getenv("FCMD_LIST_BITS"))
+        if (std::strcmp(lb, "32") == 0) s->g.list16 = 0;
     s->g.force_u = 4;
     s->g.nbr_cu = 2;
     s->g.wl_grid = 2048;

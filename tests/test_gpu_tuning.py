@@ -51,3 +51,41 @@ def test_autotune_picks_a_candidate_and_the_run_goes_on():
         s.run(20, use_graph=True)
         with pytest.raises(fcmd.FcmdError):
             s.set_tuning(3, 2, 2048)
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+def test_list_entry_width_gives_the_same_bits(precision, monkeypatch):
+    """16-bit (trajectory-relative) and 32-bit list entries hold the same
+    lists: states, energies, the reference-layout half list, the
+    largest-cluster series and the cluster labels agree bit for bit (the
+    clusters' kinetic-energy sums, accumulated with atomics, to rounding)."""
+    params, states = _small_ensemble()
+    outs = []
+    for bits in ("16", "32"):
+        if bits == "32":
+            monkeypatch.setenv("FCMD_LIST_BITS", "32")
+        else:
+            monkeypatch.delenv("FCMD_LIST_BITS", raising=False)
+        with fcmd.System(params, precision) as s:
+            W.setup(s, params, states)
+            series, _ = s.run(60, cluster_every=30, use_graph=True)
+            halves = [s.halflist(t) for t in (0, len(params) - 1)]
+            clus = [s.stoddard(0, traj=t) for t in (0, len(params) - 1)]
+            outs.append(([(s.download(t), s.scalars(t).epot, s.scalars(t).psi, s.scalars(t).nrebuild)
+                          for t in range(len(params))], halves, clus, series))
+    (sa, ha, ca, ra_), (sb, hb, cb, rb_) = outs
+    np.testing.assert_array_equal(ra_, rb_)
+    for (a, ea, pa, ra), (b, eb, pb, rb) in zip(sa, sb):
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+            np.testing.assert_array_equal(a[k], b[k])
+        assert (ea, pa, ra) == (eb, pb, rb)
+    for (la, ia), (lb, ib) in zip(ha, hb):
+        np.testing.assert_array_equal(la, lb)
+        np.testing.assert_array_equal(ia, ib)
+    for x, y in zip(ca, cb):
+        for k in x:
+            u, v = np.asarray(x[k]), np.asarray(y[k])
+            if u.dtype.kind == "f":  # kinetic-energy sums: atomics, order-free to rounding
+                np.testing.assert_allclose(u, v, rtol=1e-12, atol=0)
+            else:
+                np.testing.assert_array_equal(u, v)
