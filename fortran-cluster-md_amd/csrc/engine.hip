@@ -105,6 +105,9 @@ struct fcmd_system {
     DevPtrs d{};
     std::vector<void *> allocs;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                  // forces beside the rebuild (enqueue_step)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int overlap = 1;                             // FCMD_OVERLAP=0: one stream, no side launches
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *stage = nullptr;    // natom-sized staging (device)
     int *stage_i = nullptr;
@@ -131,6 +134,9 @@ struct fcmd_system {
         for (void *p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -234,11 +240,31 @@ int sync_err(fcmd_system *s)
 // one integrator step of every trajectory.  The Nose-Hoover final velocity
 // update is left pending and done by the next drift; finalize() applies it
 // wherever v is read or replaced outside the step sequence.
-void enqueue_step(fcmd_system *s)
+//
+// The rebuild is a chain of small dependent kernels over the ~1/nbtim of the
+// atoms whose trajectories rebuild now; it leaves most of the GPU idle.  So
+// the forces of the trajectories that keep their lists (which the rebuild
+// does not touch) run on the side stream beside it, and only the rebuilt
+// trajectories' forces follow it.  Every atom block is computed by the same
+// kernel code either way, so the results are the same bits as the forces of
+// all blocks in one launch after the rebuild (s->overlap = 0).
+int enqueue_step(fcmd_system *s)
 {
     launch_drift(s->d, s->g, s->mixed, s->stream);
+    if (!s->overlap) {
+        launch_rebuild(s->d, s->g, s->mixed, s->stream);
+        launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        return FCMD_OK;
+    }
+    HIPCHK(hipEventRecord(s->ev_fork, s->stream));
+    HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->side, {}, FP_KEPT);
+    HIPCHK(hipEventRecord(s->ev_join, s->side));
     launch_rebuild(s->d, s->g, s->mixed, s->stream);
-    launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_REBUILT);
+    HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    launch_traj_post(s->d, s->g, FORCE_STEP, s->stream);
+    return FCMD_OK;
 }
 
 void finalize(fcmd_system *s) { launch_kick_nh(s->d, s->g, s->mixed, s->stream); }
@@ -390,8 +416,12 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         wsz[WL_LISTS * k + WL_CCHK] = t.ncblk;
         wsz[WL_LISTS * k + WL_TRAJ] = 1;
     }
+    if (const char *ov = std::getenv("FCMD_OVERLAP"))
+        if (std::strcmp(ov, "0") == 0) s->overlap = 0;
     if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
-        hipEventCreate(&s->ev1) != hipSuccess) {
+        hipEventCreate(&s->ev1) != hipSuccess || hipStreamCreate(&s->side) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete s;
         return FCMD_ERR_HIP;
     }
@@ -584,7 +614,7 @@ int fcmd_integrator(fcmd_system *s, int *step)
     for (int k = 0; k < s->ntraj; ++k)
         HIPCHK(hipMemcpyAsync((char *)(s->d.ts + k) + offsetof(TrajScalars, step), step, sizeof(int),
                               hipMemcpyHostToDevice, s->stream));
-    enqueue_step(s);
+    if (int rc = enqueue_step(s)) return rc;
     finalize(s);  // integrator(step) returns with the step's final velocities
     HIPCHK(hipGetLastError());
     return sync_err(s);
@@ -733,8 +763,13 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
         }
         hipGraph_t gr;
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < gsteps; ++k) enqueue_step(s);
+        int rc = FCMD_OK;
+        for (int k = 0; k < gsteps && !rc; ++k) rc = enqueue_step(s);
         HIPCHK(hipStreamEndCapture(s->stream, &gr));
+        if (rc) {
+            (void)hipGraphDestroy(gr);
+            return rc;
+        }
         HIPCHK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(gr));
         s->graph_steps = gsteps;
@@ -761,7 +796,8 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
         int k = 0;
         if (use_graph)
             for (; k + s->graph_steps <= todo; k += s->graph_steps) HIPCHK(hipGraphLaunch(s->graph, s->stream));
-        for (; k < todo; ++k) enqueue_step(s);
+        for (; k < todo; ++k)
+            if (int rc = enqueue_step(s)) return rc;
         HIPCHK(hipGetLastError());
         done += todo;
     }

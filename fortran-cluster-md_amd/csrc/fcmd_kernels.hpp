@@ -11,6 +11,10 @@ constexpr int SCAN_CHUNK = 4 * BLK;  // cells per scan chunk
 
 enum { FORCE_STEP = 0, FORCE_ACCATOM = 1 };
 
+// which atom blocks a force launch takes: all; those of trajectories that
+// keep their list this step; those of trajectories that rebuild (work list)
+enum { FP_ALL = 0, FP_KEPT = 1, FP_REBUILT = 2 };
+
 struct LaunchDims {
     int nblk;    // atom blocks
     int ncblk;   // cell chunks
@@ -48,7 +52,7 @@ void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s, const
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
                   hipStream_t s);
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                        hipStream_t s, const KTimer &kt = {});
+                        hipStream_t s, const KTimer &kt = {}, int part = FP_ALL);
 void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt = {});
 // applies any pending Nose-Hoover final update and clears the flags
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);

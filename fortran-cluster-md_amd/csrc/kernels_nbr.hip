@@ -265,53 +265,66 @@ constexpr int CS_REG = 8;
 
 // sort of each cell's atoms by reference atom number, so the order (and every
 // later floating-point sum) does not depend on atomics; also writes each
-// cell's final start.  One cell per thread (SCAN_CHUNK threads per chunk).
-// A cell of up to CS_REG atoms loads its members and their numbers at once
-// and sorts them in registers (odd-even transposition, CS_REG phases);
-// a larger one is sorted in place.
-__global__ __launch_bounds__(SCAN_CHUNK) void k_cellsort(DevPtrs d)
+// cell's final start.  One BLK-thread workgroup per chunk, each thread taking
+// the chunk's cells threadIdx.x + q*BLK (q < CS_PER): a workgroup no wider
+// than the force kernel's, so that it finds room on a CU beside the force
+// launch that runs during the rebuild.  A cell of up to CS_REG atoms loads
+// its members and their numbers at once and sorts them in registers
+// (odd-even transposition, CS_REG phases); a larger one is sorted in place.
+constexpr int CS_PER = SCAN_CHUNK / BLK;  // cells per thread
+
+__device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
+{
+    if (n < 2) return;
+    if (n <= CS_REG) {
+        int v[CS_REG], key[CS_REG];
+#pragma unroll
+        for (int p = 0; p < CS_REG; ++p) v[p] = p < n ? a[p] : 0;
+#pragma unroll
+        for (int p = 0; p < CS_REG; ++p) key[p] = p < n ? d.oid[v[p]] : INT_MAX;
+#pragma unroll
+        for (int ph = 0; ph < CS_REG; ++ph)
+#pragma unroll
+            for (int p = ph & 1; p + 1 < CS_REG; p += 2) cmp_swap(key[p], v[p], key[p + 1], v[p + 1]);
+#pragma unroll
+        for (int p = 0; p < CS_REG; ++p)
+            if (p < n) a[p] = v[p];
+    } else {
+        for (int p = 1; p < n; ++p) {
+            const int v = a[p];
+            const int key = d.oid[v];
+            int q = p - 1;
+            while (q >= 0 && d.oid[a[q]] > key) {
+                a[q + 1] = a[q];
+                --q;
+            }
+            a[q + 1] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
 {
     FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
         const TrajParams &tp = d.tp[d.cblk_traj[c]];
         const int add = (int)d.scan_tmp_ll[c];
-        const int cell = c * SCAN_CHUNK + threadIdx.x;
-        // chunk-relative start and end (the next cell's start; for the
-        // chunk's last cell, the next chunk's offset, or natom)
-        const int s0 = d.cell_start[cell];
-        const int e0 = threadIdx.x + 1 < SCAN_CHUNK ? d.cell_start[cell + 1]
-                       : (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
-        __syncthreads();  // every chunk-relative start is read before any is replaced
-        const int start = s0 + add;
-        d.cell_start[cell] = start;
-        const int n = e0 - s0;
-        if (n < 2) continue;
-        int *a = d.sorted + tp.abase + start;
-        if (n <= CS_REG) {
-            int v[CS_REG], key[CS_REG];
+        // the chunk's end: the next chunk's offset, or natom
+        const int cend = (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
+        // chunk-relative start and end (the next cell's start) of each cell
+        int s0[CS_PER], e0[CS_PER];
 #pragma unroll
-            for (int p = 0; p < CS_REG; ++p) v[p] = p < n ? a[p] : 0;
-#pragma unroll
-            for (int p = 0; p < CS_REG; ++p) key[p] = p < n ? d.oid[v[p]] : INT_MAX;
-#pragma unroll
-            for (int ph = 0; ph < CS_REG; ++ph)
-#pragma unroll
-                for (int p = ph & 1; p + 1 < CS_REG; p += 2) cmp_swap(key[p], v[p], key[p + 1], v[p + 1]);
-#pragma unroll
-            for (int p = 0; p < CS_REG; ++p)
-                if (p < n) a[p] = v[p];
-        } else {
-            for (int p = 1; p < n; ++p) {
-                const int v = a[p];
-                const int key = d.oid[v];
-                int q = p - 1;
-                while (q >= 0 && d.oid[a[q]] > key) {
-                    a[q + 1] = a[q];
-                    --q;
-                }
-                a[q + 1] = v;
-            }
+        for (int q = 0; q < CS_PER; ++q) {
+            const int local = threadIdx.x + q * BLK;
+            const int cell = c * SCAN_CHUNK + local;
+            s0[q] = d.cell_start[cell];
+            e0[q] = local + 1 < SCAN_CHUNK ? d.cell_start[cell + 1] : cend;
         }
+        __syncthreads();  // every chunk-relative start is read before any is replaced
+#pragma unroll
+        for (int q = 0; q < CS_PER; ++q) d.cell_start[c * SCAN_CHUNK + threadIdx.x + q * BLK] = s0[q] + add;
+#pragma unroll 1
+        for (int q = 0; q < CS_PER; ++q) sort_cell(d, d.sorted + tp.abase + s0[q] + add, e0[q] - s0[q]);
     }
 }
 
@@ -593,7 +606,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, gall, blk, 0, s, d);
-    hipLaunchKernelGGL(k_cellsort, gc, dim3(SCAN_CHUNK), 0, s, d);
+    hipLaunchKernelGGL(k_cellsort, gc, blk, 0, s, d);
     if (mixed)
         hipLaunchKernelGGL(k_permute<float>, gall, blk, 0, s, d);
     else

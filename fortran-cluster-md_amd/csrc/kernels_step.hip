@@ -606,19 +606,34 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
 // LT: list entry type.  Single-species kernels of up to 4 neighbours a
 // batch are held to 72 VGPRs (7 wavefronts a SIMD; without spills), which
 // the 16-bit list decode would otherwise just exceed.
+//
+// part: which atom blocks the launch takes.  FP_ALL: every block.
+// FP_KEPT: the blocks of trajectories that keep their list this step (this
+// launch can run beside the rebuild of the others).  FP_REBUILT: the blocks
+// of the trajectories that rebuild, from the rebuild's work list (after it).
 template <typename AccT, bool MULTI, int U, typename LT>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U <= 4 ? 7 : 1))) void k_force(
-    DevPtrs d, int mode)
+    DevPtrs d, int mode, int part)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
     __shared__ double lds[4];
     __shared__ unsigned stage[STAGE][BLK];  // each thread's first list slots
-    const int b = xcd_block(blockIdx.x, d.nblk);
-    if (b >= d.nblk) return;
+    int b;
+    if (part == FP_REBUILT) {
+        // one work-list item per workgroup, split over the XCDs in runs
+        const int n = d.wl_cnt[WL_BLK], c = (n + NXCD - 1) / NXCD;
+        const int w = (int)(blockIdx.x % NXCD) * c + (int)(blockIdx.x / NXCD);
+        if ((int)(blockIdx.x / NXCD) >= c || w >= n) return;
+        b = d.wl_blk[w];
+    } else {
+        b = xcd_block(blockIdx.x, d.nblk);
+        if (b >= d.nblk) return;
+    }
     const int t = d.blk_traj[b];
     const TrajParams &tp = d.tp[t];
     const TrajScalars &ts = d.ts[t];
     if (ts.err) return;
+    if (part == FP_KEPT && ts.rebuild) return;
     const int i = b * BLK + threadIdx.x;
     const int rel = i - tp.abase;
     const bool act = rel < tp.natom;
@@ -785,39 +800,41 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
 }
 
 template <typename AccT, bool MULTI, typename LT>
-static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt)
+static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
+                            const KTimer &kt)
 {
     const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
     if (g.force_u == 2)
-        launch_timed(k_force<AccT, MULTI, 2, LT>, grid, blk, s, kt, d, mode);
+        launch_timed(k_force<AccT, MULTI, 2, LT>, grid, blk, s, kt, d, mode, part);
     else if (g.force_u == 8)
-        launch_timed(k_force<AccT, MULTI, 8, LT>, grid, blk, s, kt, d, mode);
+        launch_timed(k_force<AccT, MULTI, 8, LT>, grid, blk, s, kt, d, mode, part);
     else
-        launch_timed(k_force<AccT, MULTI, 4, LT>, grid, blk, s, kt, d, mode);
+        launch_timed(k_force<AccT, MULTI, 4, LT>, grid, blk, s, kt, d, mode, part);
 }
 
 template <typename AccT, bool MULTI>
-static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt)
+static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
+                           const KTimer &kt)
 {
     if (g.list16)
-        launch_force_lt<AccT, MULTI, unsigned short>(d, g, mode, s, kt);
+        launch_force_lt<AccT, MULTI, unsigned short>(d, g, mode, part, s, kt);
     else
-        launch_force_lt<AccT, MULTI, int>(d, g, mode, s, kt);
+        launch_force_lt<AccT, MULTI, int>(d, g, mode, part, s, kt);
 }
 
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                        hipStream_t s, const KTimer &kt)
+                        hipStream_t s, const KTimer &kt, int part)
 {
     if (mixed) {
         if (single_species)
-            launch_force_u<float, false>(d, g, mode, s, kt);
+            launch_force_u<float, false>(d, g, mode, part, s, kt);
         else
-            launch_force_u<float, true>(d, g, mode, s, kt);
+            launch_force_u<float, true>(d, g, mode, part, s, kt);
     } else {
         if (single_species)
-            launch_force_u<double, false>(d, g, mode, s, kt);
+            launch_force_u<double, false>(d, g, mode, part, s, kt);
         else
-            launch_force_u<double, true>(d, g, mode, s, kt);
+            launch_force_u<double, true>(d, g, mode, part, s, kt);
     }
 }
 

@@ -89,3 +89,26 @@ def test_list_entry_width_gives_the_same_bits(precision, monkeypatch):
                 np.testing.assert_allclose(u, v, rtol=1e-12, atol=0)
             else:
                 np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_forces_beside_the_rebuild_give_the_same_bits(use_graph, monkeypatch):
+    """The step runs the forces of the trajectories that keep their lists on
+    a side stream beside the rebuild of the others; the same step on one
+    stream (FCMD_OVERLAP=0) must give the same bits, graph or not."""
+    params, states = _small_ensemble()
+    outs = []
+    for ov in ("1", "0"):
+        monkeypatch.setenv("FCMD_OVERLAP", ov)
+        with fcmd.System(params, fcmd.MIXED) as s:
+            W.setup(s, params, states)
+            series, _ = s.run(80, cluster_every=40, use_graph=use_graph)
+            outs.append(([(s.download(t), s.scalars(t).epot, s.scalars(t).psi, s.scalars(t).nrebuild)
+                          for t in range(len(params))], series))
+    (sa, ra_), (sb, rb_) = outs
+    np.testing.assert_array_equal(ra_, rb_)
+    assert sum(r for *_, r in sa) > len(params)  # the run did rebuild
+    for (a, ea, pa, ra), (b, eb, pb, rb) in zip(sa, sb):
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+            np.testing.assert_array_equal(a[k], b[k])
+        assert (ea, pa, ra) == (eb, pb, rb)
