@@ -34,6 +34,33 @@ __device__ __forceinline__ void unpack_cell(float w, int &cx, int &cy, int &cz)
     cz = c >> (2 * CELL_BITS);
 }
 
+// the cell coordinates of a packed cell as floats (one byte conversion each)
+__device__ __forceinline__ float cell_xf(float w) { return (float)(__float_as_uint(w) & 0xffu); }
+__device__ __forceinline__ float cell_yf(float w) { return (float)((__float_as_uint(w) >> 8) & 0xffu); }
+__device__ __forceinline__ float cell_zf(float w) { return (float)((__float_as_uint(w) >> 16) & 0xffu); }
+
+// One coordinate of a pair separation from cell-offset records, on a grid of
+// nc >= 3 cells a side (inc = 1/nc): the cell difference dc = ci - cj of two
+// cells that are neighbours modulo nc is -1, 0, 1 or +-(nc - 1); dc/nc rounds
+// to the periodic image (0 or +-1), so dc - nc*rint(dc/nc) is the wrapped
+// difference, exactly (small integers).  Then dc*side + (off_i - off_j).
+// FP32 throughout; the two roundings of the fused multiply-adds are the
+// mixed path's.
+__device__ __forceinline__ float sep_wrap(float ci, float cj, float nc, float inc, float side, float doff)
+{
+    const float dc = ci - cj;
+    const float dw = __builtin_fmaf(-nc, rintf(dc * inc), dc);
+    return __builtin_fmaf(dw, side, doff);
+}
+
+// the same on a side of fewer than 3 cells: the cell difference, then the
+// remaining image by rounding
+__device__ __forceinline__ float sep_small(float ci, float cj, float side, float doff, float box, float ibox)
+{
+    const float s = __builtin_fmaf(ci - cj, side, doff);
+    return __builtin_fmaf(-box, rintf(s * ibox), s);
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

@@ -25,7 +25,9 @@ constexpr int MAXS = 10;    // species, const.inc:12 mxsorts
 constexpr int EMAX = 48;    // cluster edges (r <= atskin) kept per atom
 constexpr int NPART = 4;    // per-block partial sums
 constexpr int NXCD = 8;     // XCDs of one MI355X, each with its own L2
-constexpr int CELL_BITS = 10;                   // per coordinate in a packed cell
+// per coordinate in a packed cell: a byte, which one v_cvt_f32_ubyte turns
+// into a float (the pair loops work with cell coordinates as floats)
+constexpr int CELL_BITS = 8;
 constexpr int CELL_MAX = (1 << CELL_BITS) - 1;  // cells per box side at most
 
 // XCD-aware block order.  Workgroup IDs are handed to the XCDs round-robin;

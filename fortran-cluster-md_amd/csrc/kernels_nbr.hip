@@ -373,18 +373,6 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 // The reference's pair test on the permuted FP64 positions: on wrapped
 // positions, d = x(i) - x(j) with i the lower reference atom number,
 // d = d + L - L*int(1.5 + d/L), r2 <= atrskin^2 (neighbor-pbc.f:79-84)
-// one coordinate of a separation from two cell-offset records (the cell
-// difference wrapped to a neighbour; below 3 cells a side, the rounding image)
-__device__ __forceinline__ float sep_cells(int ci, int cj, int nc, float side, float doff, float box,
-                                           float ibox)
-{
-    int dc = ci - cj;
-    dc += dc > 1 ? -nc : (dc < -1 ? nc : 0);
-    float s = (float)dc * side + doff;
-    if (nc < 3) s -= box * rintf(s * ibox);
-    return s;
-}
-
 __device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, const TrajParams &tp, int i, int j,
                                                  double rs2)
 {
@@ -417,7 +405,13 @@ __device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, const Traj
 // second range following its first), CU candidates (a tunable) per iteration
 // with their loads issued together.  The visiting order is the same as cell
 // by cell.
-template <int CU, typename Sink>
+//
+// Separations: FP32 from the cell-offset records.  On a grid of at least 3
+// cells a side (!SMALL) the candidate's y and z cell offsets are the row's
+// and the plane's (-1, 0 or 1), and x is wrapped from the records' cells
+// (sep_wrap); with fewer than 3 cells on some side (SMALL) every coordinate
+// takes the records' cells and, on such a side, the rounding image.
+template <int CU, bool SMALL, typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                           int i, int &half, Sink sink)
 {
@@ -431,6 +425,10 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
     int cx, cy, cz;
     unpack_cell(pi.w, cx, cy, cz);
+    const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
+    const float ncxf = (float)ncx, ncyf = (float)ncy, nczf = (float)ncz;
+    // 1/n to 1 ulp: dc/n only has to round to the right integer
+    const float icx = __builtin_amdgcn_rcpf(ncxf), icy = __builtin_amdgcn_rcpf(ncyf), icz = __builtin_amdgcn_rcpf(nczf);
     const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
     const float bxf = (float)tp.box[0], byf = (float)tp.box[1], bzf = (float)tp.box[2];
     const float ibxf = (float)tp.ibox[0], ibyf = (float)tp.ibox[1], ibzf = (float)tp.ibox[2];
@@ -484,9 +482,11 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
         // the plane's three rows walked as one sequence (lanes whose rows
         // differ in length stay busy across row ends)
         const int e0 = len[0], e1 = e0 + len[1], n = e1 + len[2];
+        const float mozf = -(float)oz;  // wrapped cell difference cz_i - cz_j of the plane
         for (int k0 = 0; k0 < n; k0 += CU) {
             int jj[CU];
             float4 pj[CU];
+            float moyf[CU];
 #pragma unroll
             for (int u = 0; u < CU; ++u) {
                 const int k = min(k0 + u, n - 1);
@@ -497,18 +497,28 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                 const int bb = r0 ? B[0] : (r1 ? B[1] : B[2]);
                 jj[u] = kr < na ? ba + kr : bb + kr;
                 pj[u] = d.posf[jj[u]];
+                moyf[u] = r0 ? 1.0f : (r1 ? 0.0f : -1.0f);  // rows oy = -1, 0, 1 (!SMALL)
             }
 #pragma unroll
             for (int u = 0; u < CU; ++u) {
                 if (k0 + u >= n) break;
                 const int j = jj[u];
                 if (j == i) continue;
-                int cxj, cyj, czj;
-                unpack_cell(pj[u].w, cxj, cyj, czj);
-                const float fx = sep_cells(cx, cxj, ncx, sx, pi.x - pj[u].x, bxf, ibxf);
-                const float fy = sep_cells(cy, cyj, ncy, sy, pi.y - pj[u].y, byf, ibyf);
-                const float fz = sep_cells(cz, czj, ncz, sz, pi.z - pj[u].z, bzf, ibzf);
-                const float r2f = fx * fx + fy * fy + fz * fz;
+                float fx, fy, fz;
+                if (SMALL) {
+                    const float cxj = cell_xf(pj[u].w), cyj = cell_yf(pj[u].w), czj = cell_zf(pj[u].w);
+                    fx = ncx < 3 ? sep_small(cxf, cxj, sx, pi.x - pj[u].x, bxf, ibxf)
+                                 : sep_wrap(cxf, cxj, ncxf, icx, sx, pi.x - pj[u].x);
+                    fy = ncy < 3 ? sep_small(cyf, cyj, sy, pi.y - pj[u].y, byf, ibyf)
+                                 : sep_wrap(cyf, cyj, ncyf, icy, sy, pi.y - pj[u].y);
+                    fz = ncz < 3 ? sep_small(czf, czj, sz, pi.z - pj[u].z, bzf, ibzf)
+                                 : sep_wrap(czf, czj, nczf, icz, sz, pi.z - pj[u].z);
+                } else {
+                    fx = sep_wrap(cxf, cell_xf(pj[u].w), ncxf, icx, sx, pi.x - pj[u].x);
+                    fy = __builtin_fmaf(moyf[u], sy, pi.y - pj[u].y);
+                    fz = __builtin_fmaf(mozf, sz, pi.z - pj[u].z);
+                }
+                const float r2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
                 bool in = r2f <= in2;
                 if (!in && r2f <= out2) in = reference_pair_test(d, tp, i, j, rs2);
                 if (in) {
@@ -545,9 +555,13 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         int cnt = 0, half = 0;
         if (rel < tp.natom) {
             LT *out = list_base<LT>(d) + list_slot(tp, rel);
-            cnt = candidates<CU>(d, tp, ts, i, half, [&](int j, int k) {
+            auto sink = [&](int j, int k) {
                 if (k < width) __builtin_nontemporal_store((LT)(j - tp.abase), out + (size_t)k * TILE);
-            });
+            };
+            if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
+                cnt = candidates<CU, true>(d, tp, ts, i, half, sink);
+            else
+                cnt = candidates<CU, false>(d, tp, ts, i, half, sink);
         }
         if (cnt > width) {
             ts.err = ERR_LISTFULL;

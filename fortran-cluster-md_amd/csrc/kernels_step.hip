@@ -503,39 +503,28 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
     walk_list<U>(stage, h, load, sum);
 }
 
-// one coordinate of a mixed-path separation: cell difference wrapped to a
-// neighbour (every listed pair was in neighbouring cells at the list build),
-// then, when SMALL (some side of the box has fewer than 3 cells), the
-// remaining image by rounding on the sides that need it
-template <bool SMALL>
-__device__ __forceinline__ float sep_mixed(int ci, int cj, int nc, float side, float off, float box,
-                                           float ibox)
-{
-    int dc = ci - cj;
-    dc += dc > 1 ? -nc : (dc < -1 ? nc : 0);
-    float s = (float)dc * side + off;
-    if (SMALL && nc < 3) s -= box * rintf(s * ibox);
-    return s;
-}
-
 // Mixed: separations from the FP32 cell-offset records (make_posf: one 16-byte
-// gather per neighbour, no FP64), pair distance and LJ terms in FP32 (1/r^2 by
-// the hardware reciprocal, 1 ulp), forces summed in FP32 with Kahan
-// compensation, energies summed in FP64.  SMALL: a separate loop for boxes
-// with fewer than 3 cells on some side, so the usual case carries no rounding
-// image.
+// gather per neighbour, no FP64): the cell difference wrapped to a neighbour
+// (sep_wrap; every listed pair was in neighbouring cells at the list build),
+// or for a box of fewer than 3 cells on some side (SMALL, a loop of its own)
+// the rounding image.  Pair distance and LJ terms in FP32 with fused
+// multiply-adds (1/r^2 by the hardware reciprocal, 1 ulp); forces summed in
+// FP32 with Kahan compensation, energies summed in FP64.  The pair constants
+// of the single-species form are held in registers.
 template <int FORM, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
                                                double &fy, double &fz, double &e, unsigned (*stage)[BLK])
 {
-    int cxi, cyi, czi;
-    unpack_cell(pi.w, cxi, cyi, czi);
-    const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
+    const float cxi = cell_xf(pi.w), cyi = cell_yf(pi.w), czi = cell_zf(pi.w);
+    const float ncx = (float)ts.ncx, ncy = (float)ts.ncy, ncz = (float)ts.ncz;
+    // 1/n to 1 ulp: dc/n only has to round to the right integer
+    const float icx = __builtin_amdgcn_rcpf(ncx), icy = __builtin_amdgcn_rcpf(ncy), icz = __builtin_amdgcn_rcpf(ncz);
     const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
     const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
     const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
     const float rc2f = (float)(tp.atrcut * tp.atrcut);
+    const float c1 = tp.c1f[0], c2 = tp.c2f[0], c3 = tp.c3f[0];
     const int nn = h.nn;
     KahanF kx, ky, kz;
     int jj[U], tj[U];
@@ -552,23 +541,32 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (s0 + u >= nn) break;
-            int cxj, cyj, czj;
-            unpack_cell(pj[u].w, cxj, cyj, czj);
-            const float dxf = sep_mixed<SMALL>(cxi, cxj, ncx, sx, pi.x - pj[u].x, bx, ibx);
-            const float dyf = sep_mixed<SMALL>(cyi, cyj, ncy, sy, pi.y - pj[u].y, by, iby);
-            const float dzf = sep_mixed<SMALL>(czi, czj, ncz, sz, pi.z - pj[u].z, bz, ibz);
-            const float r2 = dxf * dxf + dyf * dyf + dzf * dzf;
+            const float cxj = cell_xf(pj[u].w), cyj = cell_yf(pj[u].w), czj = cell_zf(pj[u].w);
+            float dxf, dyf, dzf;
+            if (SMALL) {
+                dxf = ncx < 3.0f ? sep_small(cxi, cxj, sx, pi.x - pj[u].x, bx, ibx)
+                                 : sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - pj[u].x);
+                dyf = ncy < 3.0f ? sep_small(cyi, cyj, sy, pi.y - pj[u].y, by, iby)
+                                 : sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - pj[u].y);
+                dzf = ncz < 3.0f ? sep_small(czi, czj, sz, pi.z - pj[u].z, bz, ibz)
+                                 : sep_wrap(czi, czj, ncz, icz, sz, pi.z - pj[u].z);
+            } else {
+                dxf = sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - pj[u].x);
+                dyf = sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - pj[u].y);
+                dzf = sep_wrap(czi, czj, ncz, icz, sz, pi.z - pj[u].z);
+            }
+            const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
             if (r2 <= rc2f) {
                 const float ri2 = __builtin_amdgcn_rcpf(r2);
                 const float r6 = ri2 * ri2 * ri2;
                 float fij;
                 if (FORM == ACC_1) {
-                    fij = ri2 * r6 * (tp.c2f[0] * r6 - tp.c3f[0]);
-                    e += (double)(r6 * (tp.c2f[0] * r6 - tp.c1f[0]));
+                    fij = ri2 * r6 * __builtin_fmaf(c2, r6, -c3);
+                    e += (double)(r6 * __builtin_fmaf(c2, r6, -c1));
                 } else {
                     const int pij = ti * MAXS + tj[u];
-                    fij = (float)tp.e48[pij] * ri2 * r6 * (tp.c2f[pij] * r6 - tp.c3f[pij]);
-                    e += tp.e4[pij] * (double)(r6 * (tp.c2f[pij] * r6 - tp.c1f[pij]));
+                    fij = (float)tp.e48[pij] * ri2 * r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
+                    e += tp.e4[pij] * (double)(r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]));
                 }
                 kx.add(fij * dxf);
                 ky.add(fij * dyf);
