@@ -107,7 +107,10 @@ struct fcmd_system {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                  // forces beside the rebuild (enqueue_step)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    int overlap = 1;                             // FCMD_OVERLAP=0: one stream, no side launches
+    // forces beside the rebuild: for ensembles (one trajectory either
+    // rebuilds or not, so it has nothing to overlap); FCMD_OVERLAP=0 or 1
+    // in the environment decides instead
+    int overlap = 1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *stage = nullptr;    // natom-sized staging (device)
     int *stage_i = nullptr;
@@ -416,8 +419,8 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         wsz[WL_LISTS * k + WL_CCHK] = t.ncblk;
         wsz[WL_LISTS * k + WL_TRAJ] = 1;
     }
-    if (const char *ov = std::getenv("FCMD_OVERLAP"))
-        if (std::strcmp(ov, "0") == 0) s->overlap = 0;
+    s->overlap = ntraj > 1;
+    if (const char *ov = std::getenv("FCMD_OVERLAP")) s->overlap = std::strcmp(ov, "0") != 0;
     if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
         hipEventCreate(&s->ev1) != hipSuccess || hipStreamCreate(&s->side) != hipSuccess ||
         hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
