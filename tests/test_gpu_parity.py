@@ -259,3 +259,43 @@ def test_mixed_precision_small_boxes(n, rho):
     den = sum((o.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
     assert np.sqrt(num / den) < 1e-5
     assert abs(s.scalars().epot - o.st.epot) < 1e-5 * abs(o.st.epot)
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+def test_cell_grid_capped_at_255_a_side(precision):
+    """A box of more than 255 list radii a side (cell coordinates are bytes):
+    the grid keeps 255 wider cells a side, which only adds candidates.  The
+    list is still the reference's and the forces the oracle's (FP64 to 1e-12,
+    mixed to 1e-5), also after steps that rebuild on the capped grid."""
+    case = cases.make_case(n=3000, rho=4e-5, seed=71, dense_frac=0.5, ncenters=8,
+                           rc_sig=1.5, rskin_sig=1.6)
+    assert case["box"][0] / case["atrskin"] > 255
+    n = case["natom"]
+    o = po.Oracle(case, "mav")
+    o.neighbor(0)
+    o.accatom()
+    s = gpu_system(case, "mav", precision=precision)
+    s.neighbor(0)
+    lst, idx = s.halflist()
+    ref_l, ref_i = o.halflist()
+    np.testing.assert_array_equal(lst, ref_l)
+    assert len(lst) > n + 100  # the clusters give pairs
+    s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
+    s.accatom()
+    got = s.download()
+    tol = REL if precision == fcmd.FP64 else 1e-5
+    num = sum(((got[k] - o.arr[k]) ** 2).sum() for k in ("ax", "ay", "az"))
+    den = sum((o.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
+    assert np.sqrt(num / den) < tol
+    for i in range(1, 31):
+        s.integrator(i)
+    assert s.scalars().nrebuild >= 2
+    # forces on the GPU's current list and positions, against the oracle's
+    o2 = oracle_from_gpu(s, case, "mav")
+    o2.accatom()
+    s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
+    s.accatom()
+    got = s.download()
+    num = sum(((got[k] - o2.arr[k]) ** 2).sum() for k in ("ax", "ay", "az"))
+    den = sum((o2.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
+    assert np.sqrt(num / den) < tol
