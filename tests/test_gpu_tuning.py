@@ -91,8 +91,9 @@ def test_list_entry_width_gives_the_same_bits(precision, monkeypatch):
                 np.testing.assert_array_equal(u, v)
 
 
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_forces_beside_the_rebuild_give_the_same_bits(use_graph, monkeypatch):
+def test_forces_beside_the_rebuild_give_the_same_bits(use_graph, precision, monkeypatch):
     """The step runs the forces of the trajectories that keep their lists on
     a side stream beside the rebuild of the others; the same step on one
     stream (FCMD_OVERLAP=0) must give the same bits, graph or not."""
@@ -100,7 +101,7 @@ def test_forces_beside_the_rebuild_give_the_same_bits(use_graph, monkeypatch):
     outs = []
     for ov in ("1", "0"):
         monkeypatch.setenv("FCMD_OVERLAP", ov)
-        with fcmd.System(params, fcmd.MIXED) as s:
+        with fcmd.System(params, precision) as s:
             W.setup(s, params, states)
             series, _ = s.run(80, cluster_every=40, use_graph=use_graph)
             outs.append(([(s.download(t), s.scalars(t).epot, s.scalars(t).psi, s.scalars(t).nrebuild)
