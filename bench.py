@@ -11,7 +11,8 @@ scheduled cluster analyses included); data are synthetic (seeded vapour).
 Reported:
   value            mixed-precision batched path, atom-steps/s
   fp64_batched     the same run in FP64
-  fp64_plain       the plain FP64 path: one trajectory at a time (sampled)
+  fp64_plain       the plain FP64 path: one trajectory at a time (sampled,
+                   each on its own captured graph)
   speedup          value / fp64_plain
   accuracy         RMS relative force error of the mixed path vs FP64 on the
                    same state; energy-drift ratio mixed/FP64
@@ -241,7 +242,8 @@ def main():
 
     plain = None
     if not args.no_plain and len(params) > 1:
-        # one trajectory at a time, FP64, sampled every 8th trajectory
+        # one trajectory at a time, FP64, sampled every 8th trajectory; each
+        # replays its own captured 10-step graph (the warmup run captures it)
         log("[bench] plain FP64 path, one trajectory at a time")
         kp = max(20, min(K, 100))
         tot_t = 0.0
@@ -249,9 +251,9 @@ def main():
         for t in range(0, len(params), 8):
             with fcmd.System([params[t]], fcmd.FP64) as s:
                 W.setup(s, [params[t]], [states[t]])
-                s.run(5, use_graph=False)
+                s.run(10, use_graph=use_graph)
                 t0 = time.perf_counter()
-                s.run(kp, cluster_every=every if every <= kp else 0, use_graph=False)
+                s.run(kp, cluster_every=every if every <= kp else 0, use_graph=use_graph)
                 tot_t += time.perf_counter() - t0
                 tot_as += params[t].natom * kp
         plain = tot_as / tot_t
