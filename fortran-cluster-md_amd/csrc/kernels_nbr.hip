@@ -428,7 +428,8 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
     const float ncxf = (float)ncx, ncyf = (float)ncy, nczf = (float)ncz;
     // 1/n to 1 ulp: dc/n only has to round to the right integer
-    const float icx = __builtin_amdgcn_rcpf(ncxf), icy = __builtin_amdgcn_rcpf(ncyf), icz = __builtin_amdgcn_rcpf(nczf);
+    const float icx = __builtin_amdgcn_rcpf(ncxf), icy = __builtin_amdgcn_rcpf(ncyf),
+                icz = __builtin_amdgcn_rcpf(nczf);
     const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
     const float bxf = (float)tp.box[0], byf = (float)tp.box[1], bzf = (float)tp.box[2];
     const float ibxf = (float)tp.ibox[0], ibyf = (float)tp.ibox[1], ibzf = (float)tp.ibox[2];

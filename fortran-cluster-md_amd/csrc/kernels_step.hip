@@ -519,7 +519,8 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float cxi = cell_xf(pi.w), cyi = cell_yf(pi.w), czi = cell_zf(pi.w);
     const float ncx = (float)ts.ncx, ncy = (float)ts.ncy, ncz = (float)ts.ncz;
     // 1/n to 1 ulp: dc/n only has to round to the right integer
-    const float icx = __builtin_amdgcn_rcpf(ncx), icy = __builtin_amdgcn_rcpf(ncy), icz = __builtin_amdgcn_rcpf(ncz);
+    const float icx = __builtin_amdgcn_rcpf(ncx), icy = __builtin_amdgcn_rcpf(ncy),
+                icz = __builtin_amdgcn_rcpf(ncz);
     const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
     const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
     const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
