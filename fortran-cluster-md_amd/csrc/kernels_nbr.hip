@@ -331,6 +331,20 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
 // gather every per-atom array into cell order (into the alternate buffers)
 // and write each atom's cell-offset record for its cell (on the mixed path
 // also the cell alone, for the drift)
+// the acceleration array of the precision in use: FP64, or FP32 on the mixed path
+template <typename AccT>
+__device__ __forceinline__ AccT *acc_array(double *p64, float *p32)
+{
+    if constexpr (sizeof(AccT) == 8)
+        return p64;
+    else
+        return p32;
+}
+
+// Both kernels that move per-atom arrays (here and the list search's copy
+// back) issue every load before any store: the compiler cannot tell the arrays
+// apart, so a store ahead of a later load would make each array a round trip
+// of its own.
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 {
@@ -338,34 +352,34 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
     {
         const int t = d.blk_traj[b];
         const TrajParams &tp = d.tp[t];
+        const TrajScalars &ts = d.ts[t];
         const int i = b * BLK + threadIdx.x;
         const bool act = i - tp.abase < tp.natom;
         const int o = act ? d.sorted[i] : i;
+        const AccT *ax = acc_array<AccT>(d.ax, d.axf), *ay = acc_array<AccT>(d.ay, d.ayf),
+                   *az = acc_array<AccT>(d.az, d.azf);
+        const double x = d.x[o], y = d.y[o], z = d.z[o];
+        const double vx = d.vx[o], vy = d.vy[o], vz = d.vz[o];
+        const AccT a0 = ax[o], a1 = ay[o], a2 = az[o];
+        const int typ = d.typ[o], oid = d.oid[o], cw = d.atom_cell[o];
+        int cx, cy, cz;
+        unpack_cell(__int_as_float(cw), cx, cy, cz);
+        const float4 r = make_posf(x, y, z, cx, cy, cz, ts);
         if (act) {
-            const TrajScalars &ts = d.ts[t];
-            int cx, cy, cz;
-            unpack_cell(__int_as_float(d.atom_cell[o]), cx, cy, cz);
-            const float4 r = make_posf(d.x[o], d.y[o], d.z[o], cx, cy, cz, ts);
             d.posf[i] = r;
             if (sizeof(AccT) == 4) d.cellid[i] = __float_as_int(r.w);
         }
-        d.x2[i] = d.x[o];
-        d.y2[i] = d.y[o];
-        d.z2[i] = d.z[o];
-        d.vx2[i] = d.vx[o];
-        d.vy2[i] = d.vy[o];
-        d.vz2[i] = d.vz[o];
-        if (sizeof(AccT) == 8) {
-            d.ax2[i] = d.ax[o];
-            d.ay2[i] = d.ay[o];
-            d.az2[i] = d.az[o];
-        } else {
-            d.axf2[i] = d.axf[o];
-            d.ayf2[i] = d.ayf[o];
-            d.azf2[i] = d.azf[o];
-        }
-        d.typ2[i] = d.typ[o];
-        d.oid2[i] = d.oid[o];
+        d.x2[i] = x;
+        d.y2[i] = y;
+        d.z2[i] = z;
+        d.vx2[i] = vx;
+        d.vy2[i] = vy;
+        d.vz2[i] = vz;
+        acc_array<AccT>(d.ax2, d.axf2)[i] = a0;
+        acc_array<AccT>(d.ay2, d.ayf2)[i] = a1;
+        acc_array<AccT>(d.az2, d.azf2)[i] = a2;
+        d.typ2[i] = typ;
+        d.oid2[i] = oid;
     }
 }
 
@@ -551,13 +565,14 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         const TrajParams &tp = d.tp[t];
         TrajScalars &ts = d.ts[t];
         const int i = b * BLK + threadIdx.x;
-        const int rel = i - tp.abase;
+        const int abase = tp.abase;
+        const bool live = i - abase < tp.natom;
         const int width = tp.lwidth;
         int cnt = 0, half = 0;
-        if (rel < tp.natom) {
-            LT *out = list_base<LT>(d) + list_slot(tp, rel);
+        if (live) {
+            LT *out = list_base<LT>(d) + list_slot(tp, i - abase);
             auto sink = [&](int j, int k) {
-                if (k < width) __builtin_nontemporal_store((LT)(j - tp.abase), out + (size_t)k * TILE);
+                if (k < width) __builtin_nontemporal_store((LT)(j - abase), out + (size_t)k * TILE);
             };
             if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
                 cnt = candidates<CU, true>(d, tp, ts, i, half, sink);
@@ -579,29 +594,33 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
             if (red[1]) atomicAdd(&ts.npairs, red[1]);
         }
         // copy back: every slot of the block takes its permuted arrays (no
-        // thread of this kernel reads the primary ones)
-        // (streamed: nontemporal)
-#define FCMD_MOVE(a) __builtin_nontemporal_store(__builtin_nontemporal_load(d.a##2 + i), d.a + i)
-        FCMD_MOVE(x);
-        FCMD_MOVE(y);
-        FCMD_MOVE(z);
-        FCMD_MOVE(vx);
-        FCMD_MOVE(vy);
-        FCMD_MOVE(vz);
-        if (d.axf) {
-            FCMD_MOVE(axf);
-            FCMD_MOVE(ayf);
-            FCMD_MOVE(azf);
+        // thread of this kernel reads the primary ones); streamed
+        // (nontemporal), and every load issued before any store
+#define FCMD_LD(a) __builtin_nontemporal_load(d.a##2 + i)
+#define FCMD_ST(v, a) __builtin_nontemporal_store(v, d.a + i)
+        const double x = FCMD_LD(x), y = FCMD_LD(y), z = FCMD_LD(z);
+        const double vx = FCMD_LD(vx), vy = FCMD_LD(vy), vz = FCMD_LD(vz);
+        const int typ = FCMD_LD(typ), o = d.oid2[i];
+        const bool f32 = d.axf != nullptr;
+        float fa0 = 0.f, fa1 = 0.f, fa2 = 0.f;
+        double da0 = 0.0, da1 = 0.0, da2 = 0.0;
+        if (f32) {
+            fa0 = FCMD_LD(axf), fa1 = FCMD_LD(ayf), fa2 = FCMD_LD(azf);
         } else {
-            FCMD_MOVE(ax);
-            FCMD_MOVE(ay);
-            FCMD_MOVE(az);
+            da0 = FCMD_LD(ax), da1 = FCMD_LD(ay), da2 = FCMD_LD(az);
         }
-        FCMD_MOVE(typ);
-#undef FCMD_MOVE
-        const int o = d.oid2[i];
+        FCMD_ST(x, x), FCMD_ST(y, y), FCMD_ST(z, z);
+        FCMD_ST(vx, vx), FCMD_ST(vy, vy), FCMD_ST(vz, vz);
+        if (f32) {
+            FCMD_ST(fa0, axf), FCMD_ST(fa1, ayf), FCMD_ST(fa2, azf);
+        } else {
+            FCMD_ST(da0, ax), FCMD_ST(da1, ay), FCMD_ST(da2, az);
+        }
+        FCMD_ST(typ, typ);
+#undef FCMD_LD
+#undef FCMD_ST
         d.oid[i] = o;
-        if (rel < tp.natom) d.inv[tp.abase + o - 1] = i;
+        if (live) d.inv[abase + o - 1] = i;
     }
 }
 
