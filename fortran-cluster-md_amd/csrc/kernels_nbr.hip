@@ -208,27 +208,33 @@ __device__ __forceinline__ double wrap_box(double x, double L)
     return fmod(fmod(x, L) + L, L);
 }
 
-// wrap in place (neighbor-pbc.f:62-66) and bin
+// wrap in place (neighbor-pbc.f:62-66) and bin.  Every read comes before the
+// first store, and the slot's atomic before the other stores (the compiler
+// keeps no load ahead of a store it cannot tell apart).
 __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
 {
     ONE_ITEM(d.wl_blk, WL_BLK, b);
     {
-        const TrajParams &tp = d.tp[d.blk_traj[b]];
-        const TrajScalars &ts = d.ts[d.blk_traj[b]];
+        const int t = d.blk_traj[b];
+        const TrajParams &tp = d.tp[t];
+        const TrajScalars &ts = d.ts[t];
         const int i = b * BLK + threadIdx.x;
         if (i - tp.abase < tp.natom) {
-            const double x = wrap_box(d.x[i], tp.box[0]);
-            const double y = wrap_box(d.y[i], tp.box[1]);
-            const double z = wrap_box(d.z[i], tp.box[2]);
+            const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz, cbase = tp.cbase;
+            const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
+            const double icx = ts.icside[0], icy = ts.icside[1], icz = ts.icside[2];
+            const double x0 = d.x[i], y0 = d.y[i], z0 = d.z[i];  // issued together, before the wraps' branches
+            const double x = wrap_box(x0, bx), y = wrap_box(y0, by), z = wrap_box(z0, bz);
+            const int cx = cell_coord(x, icx, ncx);
+            const int cy = cell_coord(y, icy, ncy);
+            const int cz = cell_coord(z, icz, ncz);
+            const int c = (cz * ncy + cy) * ncx + cx;
+            const int slot = atomicAdd(d.cell_cnt + cbase + c, 1);
             d.x[i] = x;
             d.y[i] = y;
             d.z[i] = z;
-            const int cx = cell_coord(x, ts.icside[0], ts.ncx);
-            const int cy = cell_coord(y, ts.icside[1], ts.ncy);
-            const int cz = cell_coord(z, ts.icside[2], ts.ncz);
-            const int c = (cz * ts.ncy + cy) * ts.ncx + cx;
             d.atom_cell[i] = cx | (cy << CELL_BITS) | (cz << (2 * CELL_BITS));  // packed
-            d.atom_slot[i] = atomicAdd(d.cell_cnt + tp.cbase + c, 1);
+            d.atom_slot[i] = slot;
         }
     }
 }
@@ -241,11 +247,11 @@ __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
         const int i = b * BLK + threadIdx.x;
         if (i - tp.abase < tp.natom) {
             const TrajScalars &ts = d.ts[d.blk_traj[b]];
-            const int pc = d.atom_cell[i];
+            const int pc = d.atom_cell[i], slot = d.atom_slot[i];
             const int c = tp.cbase + ((pc >> (2 * CELL_BITS)) * ts.ncy + ((pc >> CELL_BITS) & CELL_MAX)) * ts.ncx +
                           (pc & CELL_MAX);
             const int start = d.cell_start[c] + (int)d.scan_tmp_ll[c / SCAN_CHUNK];
-            d.sorted[tp.abase + start + d.atom_slot[i]] = i;
+            d.sorted[tp.abase + start + slot] = i;
         }
     }
 }
@@ -308,7 +314,7 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
     FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
     {
         const TrajParams &tp = d.tp[d.cblk_traj[c]];
-        const int add = (int)d.scan_tmp_ll[c];
+        const int add = (int)d.scan_tmp_ll[c], abase = tp.abase;
         // the chunk's end: the next chunk's offset, or natom
         const int cend = (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
         // chunk-relative start and end (the next cell's start) of each cell
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
 #pragma unroll
         for (int q = 0; q < CS_PER; ++q) d.cell_start[c * SCAN_CHUNK + threadIdx.x + q * BLK] = s0[q] + add;
 #pragma unroll 1
-        for (int q = 0; q < CS_PER; ++q) sort_cell(d, d.sorted + tp.abase + s0[q] + add, e0[q] - s0[q]);
+        for (int q = 0; q < CS_PER; ++q) sort_cell(d, d.sorted + abase + s0[q] + add, e0[q] - s0[q]);
     }
 }
 
