@@ -193,7 +193,11 @@ int fill_params(fcmd_system *s, int k, const fcmd_params &p, TrajParams &t)
     t.dt = p.dt;
     t.Q = p.Q;
     t.att = p.att;
-    for (int a = 0; a < MAXS; ++a) t.mas[a] = a < p.nsorts ? p.mas[a] : 1.0;
+    for (int a = 0; a < MAXS; ++a) {
+        t.mas[a] = a < p.nsorts ? p.mas[a] : 1.0;
+        t.imas[a] = 1.0 / t.mas[a];
+    }
+    for (int a = 0; a < MAXS; ++a) t.mrel[a] = t.mas[a] / t.mas[0];
     pair_consts(p, t);
     t.natom = p.natom;
     t.nsorts = p.nsorts;

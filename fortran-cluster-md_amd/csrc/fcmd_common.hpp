@@ -126,17 +126,4 @@ __device__ __forceinline__ int block_sum_i(int v, int *lds4)
     return r;
 }
 
-// Kahan-compensated float accumulator (mixed-precision force sums).
-struct KahanF {
-    float s, c;
-    __device__ __forceinline__ KahanF() : s(0.0f), c(0.0f) {}
-    __device__ __forceinline__ void add(float v)
-    {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-};
-
 }  // namespace fcmd

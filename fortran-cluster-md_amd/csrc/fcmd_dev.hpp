@@ -59,6 +59,9 @@ struct TrajParams {
     double atrcut, atskin, atr, eps1; // cutoff, cluster radius, sigma and eps of species 1
     double dt, Q, att, nkt;          // nkt = 3.0d0*natom*cikb*att (int-verlet-nh-mav.f:65)
     double mas[MAXS];
+    // 1/mas[a] and mas[a]/mas[0], divided on the host (IEEE division, so the
+    // same doubles a kernel would get) to keep FP64 divisions out of the kernels
+    double imas[MAXS], mrel[MAXS];
     // pair constants, index a*MAXS+b (0-based species):
     //   c1 = sigma^6, c2 = c1*c1, c3 = 0.5*c1, e48 = 48 eps, e4 = 4 eps
     double c1[MAXS * MAXS], c2[MAXS * MAXS], c3[MAXS * MAXS];

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
             if (cpl) {
                 // reference: m1*sum(v^2) with m1 for every atom (NH is single species)
                 sv = vx * vx + vy * vy + vz * vz;
-                if (!tp.nh_refmass) sv *= tp.mas[ty] / tp.mas[0];
+                if (!tp.nh_refmass) sv *= tp.mrel[ty];
             }
             vx = vx + dth * (ax - psi * vx);
             vy = vy + dth * (ay - psi * vy);
@@ -508,9 +508,11 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
 // (sep_wrap; every listed pair was in neighbouring cells at the list build),
 // or for a box of fewer than 3 cells on some side (SMALL, a loop of its own)
 // the rounding image.  Pair distance and LJ terms in FP32 with fused
-// multiply-adds (1/r^2 by the hardware reciprocal, 1 ulp); forces summed in
-// FP32 with Kahan compensation, energies summed in FP64.  The pair constants
-// of the single-species form are held in registers.
+// multiply-adds (1/r^2 by the hardware reciprocal, 1 ulp); each pair's FP32
+// force and energy terms are summed in FP64 (an FP64 add costs what an FP32
+// one does here, and two instructions a component undercut a compensated FP32
+// sum's five).  The pair constants of the single-species form are held in
+// registers.
 template <int FORM, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
@@ -527,7 +529,7 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float rc2f = (float)(tp.atrcut * tp.atrcut);
     const float c1 = tp.c1f[0], c2 = tp.c2f[0], c3 = tp.c3f[0];
     const int nn = h.nn;
-    KahanF kx, ky, kz;
+    double sfx = 0.0, sfy = 0.0, sfz = 0.0;
     int jj[U], tj[U];
     float4 pj[U];
     auto load = [&](int s0) {
@@ -569,16 +571,16 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
                     fij = (float)tp.e48[pij] * ri2 * r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
                     e += tp.e4[pij] * (double)(r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]));
                 }
-                kx.add(fij * dxf);
-                ky.add(fij * dyf);
-                kz.add(fij * dzf);
+                sfx += (double)(fij * dxf);
+                sfy += (double)(fij * dyf);
+                sfz += (double)(fij * dzf);
             }
         }
     };
     walk_list<U>(stage, h, load, sum);
-    fx = (double)kx.s;
-    fy = (double)ky.s;
-    fz = (double)kz.s;
+    fx = sfx;
+    fy = sfy;
+    fz = sfz;
 }
 
 template <typename AccT, int FORM, int U, typename LT>
@@ -658,12 +660,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
         if (mode == FORCE_ACCATOM) load_acc<AccT>(d, i, ax0, ay0, az0);
         if (form1) {
-            const double iatm = 1.0 / tp.mas[0];
+            const double iatm = tp.imas[0];
             ax = ax0 + tp.e48[0] * fx * iatm;
             ay = ay0 + tp.e48[0] * fy * iatm;
             az = az0 + tp.e48[0] * fz * iatm;
         } else {
-            const double iatm = 1.0 / tp.mas[ti];
+            const double iatm = tp.imas[ti];
             ax = ax0 + fx * iatm;
             ay = ay0 + fy * iatm;
             az = az0 + fz * iatm;
@@ -689,7 +691,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
                 const double wx = vx + dth * ax, wy = vy + dth * ay, wz = vz + dth * az;
                 if (nh_coupled(tp, ti)) {
                     eks = wx * wx + wy * wy + wz * wz;
-                    if (!tp.nh_refmass) eks *= tp.mas[ti] / tp.mas[0];
+                    if (!tp.nh_refmass) eks *= tp.mrel[ti];
                 } else {
                     // not thermostatted: its final velocity is already known
                     ekfree = 0.5 * tp.mas[ti] * (wx * wx + wy * wy + wz * wz);
