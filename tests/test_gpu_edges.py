@@ -1,0 +1,125 @@
+"""GPU edge cases: the LJ dimer against its closed forms, atoms with no
+neighbours, and the reference's fatal conditions forced from the host.
+
+* Dimer (for-a-lj-n-pbc.f:95-110): V(r) = 4 eps ((s/r)^12 - (s/r)^6) and
+  a = 48 eps / m * r^-2 (s^12 r^-12 - s^6 r^-6 / 2) * dx, so V(2^(1/6) s) = -eps
+  with no force and V(s) = 0 with |F| = 24 eps / s, both on the FP64 and the
+  mixed path, on grids of 1 and 5 cells a side and across the periodic face.
+* No neighbours: forces and energy exactly zero.
+* neighbor-pbc*.f "Skin radius exceeded!!" when the largest displacement
+  since the last build exceeds the skin; the Nose-Hoover update reports
+  "did not converge" for a thermostat state with 1 + dt/2 psi <= 0, where
+  v' = (v + dt/2 a) / (1 + dt/2 psi') has no meaning (int-verlet-nh-mav.f:106-166).
+"""
+import numpy as np
+import pytest
+
+import cases
+import fcmd
+from test_gpu_parity import gpu_system
+
+pytestmark = pytest.mark.gpu
+
+SIG = cases.SIG_AR
+R_MIN = 2.0 ** (1.0 / 6.0) * SIG
+
+
+def dimer_case(box_sig, r, across):
+    """Two argon atoms at separation r (along x) in a cubic box of box_sig
+    sigma; across: the pair straddles the periodic face x = 0."""
+    L = box_sig * SIG
+    c = cases.make_case(n=2, rho=2.0 / box_sig ** 3, seed=3)
+    assert np.isclose(c["box"][0], L)
+    mid = 0.5 * L
+    if across:
+        x1, x2 = 0.3 * SIG, L - (r - 0.3 * SIG)
+    else:
+        x1, x2 = mid, mid + r
+    c["x"] = np.array([x1, x2])
+    c["y"] = np.array([mid, mid])
+    c["z"] = np.array([mid, mid])
+    for k in ("vx", "vy", "vz"):
+        c[k] = np.zeros(2)
+    return c
+
+
+def forces(case, precision):
+    s = gpu_system(case, "mav", precision=precision)
+    s.neighbor(0)
+    n = case["natom"]
+    s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
+    s.accatom()
+    return s.download(), s.scalars().epot
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+@pytest.mark.parametrize("box_sig", [5.0, 15.0])
+@pytest.mark.parametrize("across", [False, True])
+def test_lj_dimer_closed_forms(precision, box_sig, across):
+    tol = 1e-12 if precision == fcmd.FP64 else 1e-5
+    eps, m = cases.EPS_AR, cases.MAS_AR
+    scale = 24.0 * eps / (m * SIG)  # |a| at r = sigma
+    for r in (R_MIN, SIG):
+        c = dimer_case(box_sig, r, across)
+        got, epot = forces(c, precision)
+        L = c["box"][0]
+        dx = c["x"][0] - c["x"][1]
+        dx -= L * np.rint(dx / L)  # minimum image: -r, or +r across the face
+        dy = c["y"][0] - c["y"][1]
+        assert abs(abs(dx) - r) < 1e-9 * SIG
+        r_true = np.hypot(dx, dy)
+        sr6 = (SIG / r_true) ** 6
+        v_ref = 4.0 * eps * (sr6 * sr6 - sr6)
+        fij = r_true ** -2 * (sr6 * sr6 - 0.5 * sr6)
+        ax_ref = 48.0 * eps / m * fij * dx
+        if r == SIG:
+            assert v_ref == pytest.approx(0.0, abs=1e-5 * eps)
+            assert abs(ax_ref) == pytest.approx(scale, rel=1e-5)
+        else:
+            assert v_ref == pytest.approx(-eps, rel=1e-6)
+        assert abs(epot - v_ref) <= tol * eps
+        assert abs(got["ax"][0] - ax_ref) <= tol * scale
+        assert abs(got["ax"][1] + ax_ref) <= tol * scale  # Newton's third law
+        assert abs(got["az"][0]) <= tol * scale and abs(got["az"][1]) <= tol * scale
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+def test_atoms_without_neighbours_feel_nothing(precision):
+    c = cases.make_case(n=27, rho=2e-4, seed=5)
+    L = c["box"][0]
+    g = (np.arange(3) + 0.5) * L / 3  # 3x3x3 lattice, spacing > 5 sigma
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    c["x"], c["y"], c["z"] = X.ravel(), Y.ravel(), Z.ravel()
+    assert L / 3 > 2.8 * SIG
+    got, epot = forces(c, precision)
+    for k in ("ax", "ay", "az"):
+        assert np.all(got[k] == 0.0), k
+    assert epot == 0.0
+    s = gpu_system(c, "mav", precision=precision)
+    s.neighbor(0)
+    assert s.scalars().npairs == 0
+
+
+def test_skin_exceeded_is_fatal():
+    c = cases.make_case(n=300, rho=0.02, seed=7)
+    s = gpu_system(c, "mav")
+    s.neighbor(0)
+    skin = s.scalars().atrskin - c["atrcut"]
+    s.set_scalars(dmms=1.01 * skin)
+    with pytest.raises(fcmd.FcmdError, match="Skin radius exceeded") as e:
+        s.neighbor(5)
+    assert e.value.code == 1
+
+
+def test_nose_hoover_without_a_meaningful_update_is_reported():
+    c = cases.make_case(n=300, rho=0.02, seed=8)
+    s = gpu_system(c, "nose")
+    s.neighbor(0)
+    n = c["natom"]
+    s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
+    s.accatom()
+    s.integrator(1)  # an ordinary step first
+    s.set_scalars(psi=-10.0 / c["dt"])  # 1 + dt/2 psi = -4
+    with pytest.raises(fcmd.FcmdError, match="did not converge") as e:
+        s.integrator(2)
+    assert e.value.code == 3
