@@ -189,6 +189,7 @@ int fill_params(fcmd_system *s, int k, const fcmd_params &p, TrajParams &t)
     t.atrcut = p.atrcut;
     t.atskin = p.atskin;
     t.atr = p.sig[0];
+    t.sig2f = (float)(p.sig[0] * p.sig[0]);
     t.eps1 = p.eps[0];
     t.dt = p.dt;
     t.Q = p.Q;

@@ -67,6 +67,7 @@ struct TrajParams {
     double c1[MAXS * MAXS], c2[MAXS * MAXS], c3[MAXS * MAXS];
     double e48[MAXS * MAXS], e4[MAXS * MAXS];
     float c1f[MAXS * MAXS], c2f[MAXS * MAXS], c3f[MAXS * MAXS];
+    float sig2f;  // sigma^2 of species 1 (the single-species mixed pair sum)
     int natom, nsorts, nbtim, btim, atnenum, ethst_nh;
     int integrator, neighbor, accatom, stoddard;
     int abase, npad, bfirst, nblk;   // atoms, atom blocks

@@ -509,10 +509,10 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
 // or for a box of fewer than 3 cells on some side (SMALL, a loop of its own)
 // the rounding image.  Pair distance and LJ terms in FP32 with fused
 // multiply-adds (1/r^2 by the hardware reciprocal, 1 ulp); each pair's FP32
-// force and energy terms are summed in FP64 (an FP64 add costs what an FP32
-// one does here, and two instructions a component undercut a compensated FP32
-// sum's five).  The pair constants of the single-species form are held in
-// registers.
+// force and energy terms are summed in FP64 (an FP64 fused multiply-add costs
+// what an FP32 one does here, and fewer instructions than a compensated FP32
+// sum).  The single-species form works in sigma units, so that its constants
+// are instruction immediates.
 template <int FORM, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
@@ -527,7 +527,7 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
     const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
     const float rc2f = (float)(tp.atrcut * tp.atrcut);
-    const float c1 = tp.c1f[0], c2 = tp.c2f[0], c3 = tp.c3f[0];
+    const float sig2 = tp.sig2f;  // sigma^2 of the single-species form
     const int nn = h.nn;
     double sfx = 0.0, sfy = 0.0, sfz = 0.0;
     int jj[U], tj[U];
@@ -561,19 +561,25 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
             const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
             if (r2 <= rc2f) {
                 const float ri2 = __builtin_amdgcn_rcpf(r2);
-                const float r6 = ri2 * ri2 * ri2;
                 float fij;
                 if (FORM == ACC_1) {
-                    fij = ri2 * r6 * __builtin_fmaf(c2, r6, -c3);
-                    e += (double)(r6 * __builtin_fmaf(c2, r6, -c1));
+                    // in sigma units, s6 = (sigma/r)^6: r^-2 r^-6 (sigma^12 r^-6 -
+                    // sigma^6/2) = r^-2 s6 (s6 - 1/2) and r^-6 (sigma^12 r^-6 -
+                    // sigma^6) = s6 (s6 - 1); the constants are immediates
+                    const float u2 = sig2 * ri2;
+                    const float s6 = u2 * u2 * u2;
+                    fij = ri2 * s6 * (s6 - 0.5f);
+                    e += (double)(s6 * (s6 - 1.0f));
                 } else {
+                    const float r6 = ri2 * ri2 * ri2;
                     const int pij = ti * MAXS + tj[u];
                     fij = (float)tp.e48[pij] * ri2 * r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
                     e += tp.e4[pij] * (double)(r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]));
                 }
-                sfx += (double)(fij * dxf);
-                sfy += (double)(fij * dyf);
-                sfz += (double)(fij * dzf);
+                const double fd = (double)fij;
+                sfx = __builtin_fma((double)dxf, fd, sfx);
+                sfy = __builtin_fma((double)dyf, fd, sfy);
+                sfz = __builtin_fma((double)dzf, fd, sfz);
             }
         }
     };
