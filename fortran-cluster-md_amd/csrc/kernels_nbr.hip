@@ -392,12 +392,14 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 // ---------------------------------------------------------------------------
 // The reference's pair test on the permuted FP64 positions: on wrapped
 // positions, d = x(i) - x(j) with i the lower reference atom number,
-// d = d + L - L*int(1.5 + d/L), r2 <= atrskin^2 (neighbor-pbc.f:79-84)
-__device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, const TrajParams &tp, int i, int j,
-                                                 double rs2)
+// d = d + L - L*int(1.5 + d/L), r2 <= atrskin^2 (neighbor-pbc.f:79-84).
+// b, ib: box sides and their inverses, read once by the caller (read here,
+// after the list stores, they would be reloaded for every candidate).
+__device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, int i, int j, double rs2, const double (&b)[3],
+                                                 const double (&ib)[3])
 {
-    const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
-    const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
+    const double bx = b[0], by = b[1], bz = b[2];
+    const double ibx = ib[0], iby = ib[1], ibz = ib[2];
     const bool fwd = d.oid2[i] < d.oid2[j];
     const int a = fwd ? i : j, c = fwd ? j : i;
     double dx = d.x2[a] - d.x2[c];
@@ -436,6 +438,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                                           int i, int &half, Sink sink)
 {
     const double rs2 = ts.rs2;
+    const double bd[3] = {tp.box[0], tp.box[1], tp.box[2]}, ibd[3] = {tp.ibox[0], tp.ibox[1], tp.ibox[2]};
     // FP32 decision band: r2 computed from the cell-offset records is within
     // ~2e-6 relative of the reference's r2 (offsets rounded once to FP32,
     // |offset| <= a cell side), so outside +-1e-4 its verdict is the
@@ -541,7 +544,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                 }
                 const float r2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
                 bool in = r2f <= in2;
-                if (!in && r2f <= out2) in = reference_pair_test(d, tp, i, j, rs2);
+                if (!in && r2f <= out2) in = reference_pair_test(d, i, j, rs2, bd, ibd);
                 if (in) {
                     sink(j, cnt);
                     ++cnt;
