@@ -114,6 +114,29 @@ __device__ __forceinline__ double block_max(double v, double *lds4)
     return r;
 }
 
+// Several block reductions at the end of a kernel with one barrier between
+// them all: each value's wavefront result, then (w0 + w1) + (w2 + w3) (or the
+// max) by thread 0, as block_sum / block_max do one at a time, so the results
+// are the same bits.  Valid in thread 0 only; lds is not reused after.
+template <int N>
+__device__ __forceinline__ void block_reduce_end(double (&v)[N], const bool (&is_max)[N], double (*lds)[4])
+{
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = is_max[k] ? wave_max(v[k]) : wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) lds[k][w] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            v[k] = is_max[k] ? fmax(fmax(lds[k][0], lds[k][1]), fmax(lds[k][2], lds[k][3]))
+                             : (lds[k][0] + lds[k][1]) + (lds[k][2] + lds[k][3]);
+    }
+}
+
 __device__ __forceinline__ int block_sum_i(int v, int *lds4)
 {
     v = wave_sum_i(v);

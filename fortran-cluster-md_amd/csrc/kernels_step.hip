@@ -166,7 +166,7 @@ __device__ __forceinline__ void store_acc<float>(const DevPtrs &d, int i, double
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[2][4];
     const int b = xcd_block(blockIdx.x, d.nblk);
     if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
@@ -238,11 +238,11 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
         __builtin_nontemporal_store(vz, d.vz + i);
         d2 = dx * dx + dy * dy + dz * dz;
     }
-    const double bmax = block_max(d2, lds);
-    const double bsum = block_sum(sv, lds);
+    double r[2] = {d2, sv};
+    block_reduce_end<2>(r, {true, false}, lds);
     if (threadIdx.x == 0) {
-        d.part[(size_t)b * NPART + P_DMAX] = bmax;
-        d.part[(size_t)b * NPART + P_SUMV2] = bsum;
+        d.part[(size_t)b * NPART + P_DMAX] = r[0];
+        d.part[(size_t)b * NPART + P_SUMV2] = r[1];
     }
 }
 
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
     DevPtrs d, int mode, int part)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
-    __shared__ double lds[4];
+    __shared__ double lds[3][4];
     __shared__ unsigned stage[STAGE][BLK];  // each thread's first list slots
     int b;
     if (part == FP_REBUILT) {
@@ -714,13 +714,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
             }
         }
     }
-    const double be = block_sum(e, lds);
-    const double bk = block_sum(eks, lds);
-    const double bf = block_sum(ekfree, lds);
+    double r[3] = {e, eks, ekfree};
+    block_reduce_end<3>(r, {false, false, false}, lds);
     if (threadIdx.x == 0) {
-        d.part[(size_t)b * NPART + P_EPOT] = be;
-        d.part[(size_t)b * NPART + P_EKS] = bk;
-        d.part[(size_t)b * NPART + P_SUMV2] = bf;
+        d.part[(size_t)b * NPART + P_EPOT] = r[0];
+        d.part[(size_t)b * NPART + P_EKS] = r[1];
+        d.part[(size_t)b * NPART + P_SUMV2] = r[2];
     }
 }
 
