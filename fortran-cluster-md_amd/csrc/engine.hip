@@ -395,7 +395,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.part, (long long)s->g.nblk * NPART);
     A(d.wl_blk, s->g.nblk) A(d.wl_cchk, s->g.ncblk) A(d.wl_traj, ntraj) A(d.wl_cnt, WL_LISTS);
     A(d.wl_size, (long long)WL_LISTS * ntraj);
-    A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
+    A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.cell_first, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, s->g.ncblk);
     A(d.nn, N) A(d.posf, N);
     if (s->g.list16) {

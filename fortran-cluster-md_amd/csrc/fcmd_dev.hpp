@@ -143,6 +143,7 @@ struct DevPtrs {
     int *wl_size;                    // WL_LISTS per trajectory: its item counts
     // cells
     int *cell_cnt, *cell_start, *sorted;
+    int *cell_first;                 // final trajectory-relative cell starts (cell sort -> list search)
     int *atom_cell, *atom_slot;      // per slot at binning: packed cell (CELL_BITS each), slot in cell
     long long *scan_tmp_ll;          // chunk totals for segmented scans
     // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE],

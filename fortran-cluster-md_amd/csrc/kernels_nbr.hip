@@ -15,8 +15,8 @@
 //
 // Cell counts are zero outside the bin -> scan window (zeroed at allocation;
 // the scan clears every count of the trajectory's cell range after reading
-// it).  After the cellsort, cell_start holds trajectory-relative starts for
-// every cell of the range; a cell's atoms are [start[c], start[c + 1]), and
+// it).  After the cellsort, cell_first holds trajectory-relative starts for
+// every cell of the range; a cell's atoms are [first[c], first[c + 1]), and
 // the range keeps at least one cell past the grid, whose start is natom.
 //
 // The pair test is the reference's: on wrapped positions,
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
 // each SCAN_CHUNK-cell chunk (k_scan_chunk, which also clears the counts), a
 // scan of each trajectory's chunk totals (k_scan_totals); the chunk offset is
 // added where a start is first used (k_scatter reads start + offset,
-// k_cellsort writes the final start)
+// k_cellsort writes the final start to cell_first)
 __global__ __launch_bounds__(BLK) void k_scan_chunk(DevPtrs d)
 {
     __shared__ long long lds[4];
@@ -271,13 +271,14 @@ constexpr int CS_REG = 8;
 
 // sort of each cell's atoms by reference atom number, so the order (and every
 // later floating-point sum) does not depend on atomics; also writes each
-// cell's final start.  One BLK-thread workgroup per chunk, each thread taking
-// the chunk's cells threadIdx.x + q*BLK (q < CS_PER): a workgroup no wider
-// than the force kernel's, so that it finds room on a CU beside the force
-// launch that runs during the rebuild.  A cell of up to CS_REG atoms loads
-// its members and their numbers at once and sorts them in registers
-// (odd-even transposition, CS_REG phases); a larger one is sorted in place.
-constexpr int CS_PER = SCAN_CHUNK / BLK;  // cells per thread
+// cell's final start to cell_first.  One thread per cell: a chunk's
+// SCAN_CHUNK cells are CS_PARTS workgroups of BLK threads (no wider than the
+// force kernel's, so that they find room on a CU beside the force launch that
+// runs during the rebuild), and with the final starts in an array of their
+// own no thread waits for another.  A cell of up to CS_REG atoms loads its
+// members and their numbers at once and sorts them in registers (odd-even
+// transposition, CS_REG phases); a larger one is sorted in place.
+constexpr int CS_PARTS = SCAN_CHUNK / BLK;  // workgroups per chunk
 
 __device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
 {
@@ -311,26 +312,24 @@ __device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
 
 __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
 {
-    FOR_ITEMS(d.wl_cchk, WL_CCHK, c)
-    {
+    // the work list's chunks, CS_PARTS items each, split over the XCDs as
+    // FOR_ITEMS does (a chunk's parts stay on one XCD)
+    const int n = d.wl_cnt[WL_CCHK] * CS_PARTS, per = (n + NXCD - 1) / NXCD;
+    const int end = min(n, (int)(blockIdx.x % NXCD + 1) * per);
+    for (int w = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD); w < end; w += gridDim.x / NXCD) {
+        const int c = d.wl_cchk[w / CS_PARTS];
         const TrajParams &tp = d.tp[d.cblk_traj[c]];
         const int add = (int)d.scan_tmp_ll[c], abase = tp.abase;
-        // the chunk's end: the next chunk's offset, or natom
-        const int cend = (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
-        // chunk-relative start and end (the next cell's start) of each cell
-        int s0[CS_PER], e0[CS_PER];
-#pragma unroll
-        for (int q = 0; q < CS_PER; ++q) {
-            const int local = threadIdx.x + q * BLK;
-            const int cell = c * SCAN_CHUNK + local;
-            s0[q] = d.cell_start[cell];
-            e0[q] = local + 1 < SCAN_CHUNK ? d.cell_start[cell + 1] : cend;
-        }
-        __syncthreads();  // every chunk-relative start is read before any is replaced
-#pragma unroll
-        for (int q = 0; q < CS_PER; ++q) d.cell_start[c * SCAN_CHUNK + threadIdx.x + q * BLK] = s0[q] + add;
-#pragma unroll 1
-        for (int q = 0; q < CS_PER; ++q) sort_cell(d, d.sorted + abase + s0[q] + add, e0[q] - s0[q]);
+        const int local = (w % CS_PARTS) * BLK + threadIdx.x;
+        const int cell = c * SCAN_CHUNK + local;
+        // chunk-relative start and end (the next cell's start, or for the
+        // chunk's last cell the next chunk's offset or natom)
+        const int s0 = d.cell_start[cell];
+        const int e0 = local + 1 < SCAN_CHUNK
+                           ? d.cell_start[cell + 1]
+                           : (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
+        d.cell_first[cell] = s0 + add;
+        sort_cell(d, d.sorted + abase + s0 + add, e0 - s0);
     }
 }
 
@@ -469,7 +468,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
         xb1 = xa1 - ncx;
         xa1 = ncx - 1;
     }
-    const int *cstart = d.cell_start + tp.cbase;
+    const int *cstart = d.cell_first + tp.cbase;
     int cnt = 0;
 #pragma unroll 1
     for (int oz = lz; oz <= hz; ++oz) {
@@ -649,7 +648,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, gall, blk, 0, s, d);
-    hipLaunchKernelGGL(k_cellsort, gc, blk, 0, s, d);
+    hipLaunchKernelGGL(k_cellsort, dim3(xcd_grid(std::min(CS_PARTS * g.ncblk, wg))), blk, 0, s, d);
     if (mixed)
         hipLaunchKernelGGL(k_permute<float>, gall, blk, 0, s, d);
     else
