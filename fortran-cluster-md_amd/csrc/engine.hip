@@ -404,6 +404,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         A(d.list32, lbase);
     }
     A(d.edeg, N) A(d.edeg1, N) A(d.edges, N * EMAX) A(d.parent, N) A(d.parentf, N) A(d.liq, N);
+    A(d.clpart, (long long)s->g.nblk * 4);
     A(d.cs_tmp, N) A(d.cgs_tmp, N) A(d.csf_tmp, N) A(d.cek_tmp, N) A(d.cekf_tmp, N);
     A(d.liqo, N) A(d.rank, N) A(d.o_cl, N) A(d.o_cs, N) A(d.o_cgs, N) A(d.o_clf, N) A(d.o_ccf, N);
     A(d.o_cek, N);
@@ -641,7 +642,7 @@ int fcmd_stoddard(fcmd_system *s, int traj, int step, int *cl, int *cs, int *cc,
     for (int k = 0; k < s->ntraj; ++k) cls[k].want = (k == traj);
     HIPCHK(hipMemcpy(s->d.cls, cls.data(), sizeof(ClusterScalars) * s->ntraj, hipMemcpyHostToDevice));
     finalize(s);  // cluster kinetic energies need the final velocities
-    launch_cluster(s->d, s->g, fre, s->stream);
+    launch_cluster(s->d, s->g, fre, true, s->stream);
     HIPCHK(hipGetLastError());
     int rc = sync_err(s);
     if (rc) return rc;
@@ -793,7 +794,7 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
     while (done < nsteps) {
         if (nsnap > 0 && done % cluster_every == 0) {
             finalize(s);
-            launch_cluster(s->d, s->g, s->any_fre, s->stream);
+            launch_cluster(s->d, s->g, s->any_fre, false, s->stream);
             hipLaunchKernelGGL(k_record, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->d.ts,
                                s->ntraj, dseries + (size_t)snap * s->ntraj, dfpt, fpt_size, step0[0] + done);
             if (s->count_nmax > 0)

@@ -163,6 +163,7 @@ struct DevPtrs {
     ClusterScalars *cls;
     int *edeg, *edeg1, *edges, *parent, *parentf, *liq;
     int *cs_tmp, *cgs_tmp, *csf_tmp;
+    double *clpart;                  // per atom block: monomers, their KE, clusters, largest (cluster finish -> sums)
     double *cek_tmp, *cekf_tmp;
     // clusters: per reference atom (abase + oid - 1)
     int *liqo, *rank;

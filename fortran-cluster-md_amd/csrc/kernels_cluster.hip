@@ -68,26 +68,31 @@ __global__ __launch_bounds__(BLK) void k_cl_init(DevPtrs d)
     if (!d.cls[t].want || d.ts[t].err) return;
     const int i = b * BLK + threadIdx.x;
     const int rel = i - tp.abase;
+    const bool fre = tp.stoddard == ST_FRE;  // the tWF arrays are read for tWF only
     d.parent[i] = i;
-    d.parentf[i] = i;
     d.cs_tmp[i] = 0;
     d.cgs_tmp[i] = 0;
-    d.csf_tmp[i] = 0;
     d.cek_tmp[i] = 0.0;
-    d.cekf_tmp[i] = 0.0;
-    d.liq[i] = 0;
+    if (fre) {
+        d.parentf[i] = i;
+        d.csf_tmp[i] = 0;
+        d.cekf_tmp[i] = 0.0;
+        d.liq[i] = 0;
+    }
     if (rel < tp.natom) {
-        d.liqo[tp.abase + rel] = 0;
+        if (fre) d.liqo[tp.abase + rel] = 0;
         for (int k = (rel == 0 ? 0 : rel + 1); k <= rel + 1; ++k) {
             const int h = tp.hbase + k;
             d.o_csi[h] = 0;
-            d.o_csf[h] = 0;
-            d.o_csif[h] = 0;
-            d.o_cgsf[h] = 0;
-            d.o_cgsfa[h] = 0;
             d.o_ceks[h] = 0.0;
-            d.o_cekf[h] = 0.0;
-            d.o_ceksf[h] = 0.0;
+            if (fre) {
+                d.o_csf[h] = 0;
+                d.o_csif[h] = 0;
+                d.o_cgsf[h] = 0;
+                d.o_cgsfa[h] = 0;
+                d.o_cekf[h] = 0.0;
+                d.o_ceksf[h] = 0.0;
+            }
         }
     }
     if (b == tp.bfirst && threadIdx.x == 0) {
@@ -184,7 +189,9 @@ __global__ __launch_bounds__(BLK) void k_cl_hook(DevPtrs d)
     }
 }
 
-__global__ __launch_bounds__(BLK) void k_cl_count(DevPtrs d)
+// per_atom: also the reference-order label array (fcmd_stoddard reads it; a
+// device-driven run needs only the histograms and the largest clusters)
+__global__ __launch_bounds__(BLK) void k_cl_count(DevPtrs d, int per_atom)
 {
     const int b = xcd_block(blockIdx.x, d.nblk);
     if (b >= d.nblk) return;
@@ -194,11 +201,20 @@ __global__ __launch_bounds__(BLK) void k_cl_count(DevPtrs d)
     const int i = b * BLK + threadIdx.x;
     if (i - tp.abase >= tp.natom) return;
     const int ti = d.typ[i];
-    const int r = uf_find(d.parent, i);
-    d.o_cl[tp.abase + d.oid[i] - 1] = d.oid[r];
+    // an atom without edges is a component of its own (edges are symmetric:
+    // no other atom hangs under it), so it takes plain stores, not atomics
+    const bool alone = d.edeg[i] == 0;
+    const int r = alone ? i : uf_find(d.parent, i);
+    if (per_atom) d.o_cl[tp.abase + d.oid[i] - 1] = d.oid[r];
     const double ke = kin(d, tp, i, ti);
     if (tp.stoddard == ST_FRE && ti != 0) {
-        atomicAdd(d.cgs_tmp + r, 1);
+        if (alone)
+            d.cgs_tmp[i] = 1;
+        else
+            atomicAdd(d.cgs_tmp + r, 1);
+    } else if (alone) {
+        d.cs_tmp[i] = 1;
+        d.cek_tmp[i] = ke;
     } else {
         atomicAdd(d.cs_tmp + r, 1);
         atomicAdd(d.cek_tmp + r, ke);
@@ -242,7 +258,8 @@ __global__ __launch_bounds__(BLK) void k_cl_rank(DevPtrs d)
     }
 }
 
-__global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
+// per_atom: also the reference-order per-atom arrays (cs, cek, cgs, clf, ccf)
+__global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d, int per_atom)
 {
     __shared__ double ldsd[4];
     __shared__ int ldsi[4];
@@ -254,19 +271,22 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
     const int i = b * BLK + threadIdx.x;
     const bool act = i - tp.abase < tp.natom;
     ClusterScalars &c = d.cls[t];
-    // monomers dominate a vapour: their histogram row is summed per block
-    // (one atomic per block, not per atom)
+    // monomers dominate a vapour: their histogram row, the count of clusters
+    // and the largest one are per-block partials here, reduced per trajectory
+    // in a fixed order by k_cl_sums (no atomics on one address from every block)
     int mono = 0, pos = 0, big = 0;
     double mono_ke = 0.0;
     if (act) {
         const int oi = d.oid[i];
         const int o = tp.abase + oi - 1;
-        const bool root = ld_relaxed(d.parent + i) == i;
+        const bool root = d.edeg[i] == 0 || ld_relaxed(d.parent + i) == i;  // no edges: its own root
         if (root) {
             const int sz = d.cs_tmp[i];
-            d.o_cs[o] = sz;
-            d.o_cek[o] = d.cek_tmp[i];
-            d.o_cgs[o] = d.cgs_tmp[i];
+            if (per_atom) {
+                d.o_cs[o] = sz;
+                d.o_cek[o] = d.cek_tmp[i];
+                d.o_cgs[o] = d.cgs_tmp[i];
+            }
             if (sz == 1) {
                 mono = 1;
                 mono_ke = d.cek_tmp[i];
@@ -276,7 +296,7 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
             }
             pos = sz > 0;
             big = sz;
-        } else {
+        } else if (per_atom) {
             d.o_cs[o] = 0;
             d.o_cek[o] = 0.0;
             d.o_cgs[o] = 0;
@@ -285,14 +305,13 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
     const int bm = block_sum_i(mono, ldsi);
     const int bp = block_sum_i(pos, ldsi);
     const double be = block_sum(mono_ke, ldsd);
-    const int bb = wave_max_i(big);
-    if ((threadIdx.x & 63) == 0 && bb > 0) atomicMax(&c.cbig, bb);
+    const double bb = block_max((double)big, ldsd);
     if (threadIdx.x == 0) {
-        if (bm) {
-            atomicAdd(d.o_csi + tp.hbase + 1, bm);
-            atomicAdd(d.o_ceks + tp.hbase + 1, be);
-        }
-        if (bp) atomicAdd(&c.npos, bp);
+        double *q = d.clpart + (size_t)b * 4;
+        q[0] = (double)bm;
+        q[1] = be;
+        q[2] = (double)bp;
+        q[3] = bb;
     }
     if (!act || tp.stoddard != ST_FRE) return;
     const int oi = d.oid[i];
@@ -300,8 +319,10 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
     if (d.liq[i]) {
         const int rf = uf_find(d.parentf, i);
         const int label = d.rank[tp.abase + d.oid[rf] - 1] + 1;
-        d.o_clf[o] = label;
-        d.o_ccf[o] = 1;
+        if (per_atom) {
+            d.o_clf[o] = label;
+            d.o_ccf[o] = 1;
+        }
         if (rf == i) {
             const int sz = d.csf_tmp[i];
             d.o_csf[tp.hbase + label] = sz;
@@ -310,7 +331,7 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
             atomicAdd(d.o_ceksf + tp.hbase + sz, d.cekf_tmp[i]);
             atomicMax(&c.cbigf, sz);
         }
-    } else {
+    } else if (per_atom) {
         // last cluster to touch a non-liquid atom in the reference sweep: the
         // largest label among its liquid-like neighbours (0 if none)
         int lab = 0;
@@ -325,6 +346,36 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d)
         }
         d.o_clf[o] = lab;
         d.o_ccf[o] = 0;
+    }
+}
+
+// One block per trajectory: its blocks' k_cl_finish partials in a fixed order
+// (thread k takes blocks k, k+BLK, ...; then a fixed tree): the monomer row
+// of the histograms, the count of clusters and the largest one.
+__global__ __launch_bounds__(BLK) void k_cl_sums(DevPtrs d)
+{
+    __shared__ double lds[4];
+    const int t = blockIdx.x;
+    const TrajParams &tp = d.tp[t];
+    ClusterScalars &c = d.cls[t];
+    if (!c.want || d.ts[t].err) return;
+    double mono = 0.0, ke = 0.0, pos = 0.0, big = 0.0;
+    for (int k = threadIdx.x; k < tp.nblk; k += BLK) {
+        const double *q = d.clpart + (size_t)(tp.bfirst + k) * 4;
+        mono += q[0];
+        ke += q[1];
+        pos += q[2];
+        big = fmax(big, q[3]);
+    }
+    mono = block_sum(mono, lds);
+    ke = block_sum(ke, lds);
+    pos = block_sum(pos, lds);
+    big = block_max(big, lds);
+    if (threadIdx.x == 0) {
+        d.o_csi[tp.hbase + 1] += (int)mono;
+        d.o_ceks[tp.hbase + 1] += ke;
+        c.npos += (int)pos;
+        c.cbig = max(c.cbig, (int)big);
     }
 }
 
@@ -360,7 +411,7 @@ void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int war
     hipLaunchKernelGGL(k_size_counts, dim3(g.ntraj), dim3(BLK), 0, s, d, nmax, warmup, step);
 }
 
-void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t s)
+void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, bool per_atom, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cl_init, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     if (g.list16)
@@ -368,9 +419,10 @@ void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, hipStream_t
     else
         hipLaunchKernelGGL(k_cl_edges<int>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     hipLaunchKernelGGL(k_cl_hook, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_count, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_count, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d, (int)per_atom);
     if (fre) hipLaunchKernelGGL(k_cl_rank, dim3(g.ntraj), dim3(BLK), 0, s, d);
-    hipLaunchKernelGGL(k_cl_finish, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    hipLaunchKernelGGL(k_cl_finish, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d, (int)per_atom);
+    hipLaunchKernelGGL(k_cl_sums, dim3(g.ntraj), dim3(BLK), 0, s, d);
 }
 
 }  // namespace fcmd
