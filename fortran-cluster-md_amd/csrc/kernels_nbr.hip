@@ -493,18 +493,25 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
             sb[q] = cstart[row + xs0];
             eb[q] = cstart[row + xs1 + 1];
         }
-        int A[3], la[3], B[3], len[3];
+        // The plane's three rows walked as one sequence k = 0 .. n-1 (lanes
+        // whose rows differ in length stay busy across row ends): six
+        // segments, row q's first range then its second, ending at T[0..4]
+        // and n; candidate k is slot k + off[segment], the segment being the
+        // number of ends <= k (an empty segment's end equals the next one's
+        // start, and the later select wins).
+        int T[5], off[6], n = 0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const bool ok = ly + q <= hy;
-            la[q] = ok ? ea[q] - sa[q] : 0;
-            len[q] = la[q] + (ok && wrap ? eb[q] - sb[q] : 0);
-            A[q] = tp.abase + sa[q];
-            B[q] = tp.abase + sb[q] - la[q];
+            const int la = ok ? ea[q] - sa[q] : 0;
+            const int lb = ok && wrap ? eb[q] - sb[q] : 0;
+            off[2 * q] = tp.abase + sa[q] - n;
+            off[2 * q + 1] = tp.abase + sb[q] - la - n;
+            if (q < 2) T[2 * q] = n + la;
+            n += la + lb;
+            if (q < 2) T[2 * q + 1] = n;
+            else T[4] = n - lb;
         }
-        // the plane's three rows walked as one sequence (lanes whose rows
-        // differ in length stay busy across row ends)
-        const int e0 = len[0], e1 = e0 + len[1], n = e1 + len[2];
         const float mozf = -(float)oz;  // wrapped cell difference cz_i - cz_j of the plane
         for (int k0 = 0; k0 < n; k0 += CU) {
             int jj[CU];
@@ -513,14 +520,16 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
 #pragma unroll
             for (int u = 0; u < CU; ++u) {
                 const int k = min(k0 + u, n - 1);
-                const bool r0 = k < e0, r1 = k < e1;
-                const int kr = r0 ? k : (r1 ? k - e0 : k - e1);
-                const int na = r0 ? la[0] : (r1 ? la[1] : la[2]);
-                const int ba = r0 ? A[0] : (r1 ? A[1] : A[2]);
-                const int bb = r0 ? B[0] : (r1 ? B[1] : B[2]);
-                jj[u] = kr < na ? ba + kr : bb + kr;
+                const bool g1 = k >= T[1], g3 = k >= T[3];  // past row 0, past row 1
+                int o = off[0];
+                o = k >= T[0] ? off[1] : o;
+                o = g1 ? off[2] : o;
+                o = k >= T[2] ? off[3] : o;
+                o = g3 ? off[4] : o;
+                o = k >= T[4] ? off[5] : o;
+                jj[u] = k + o;
                 pj[u] = d.posf[jj[u]];
-                moyf[u] = r0 ? 1.0f : (r1 ? 0.0f : -1.0f);  // rows oy = -1, 0, 1 (!SMALL)
+                moyf[u] = g3 ? -1.0f : (g1 ? 0.0f : 1.0f);  // rows oy = -1, 0, 1 (!SMALL)
             }
 #pragma unroll
             for (int u = 0; u < CU; ++u) {
