@@ -2,7 +2,7 @@
 """Throughput of the LJ cluster-nucleation step on one MI355X.
 
 Default workload (ensemble512): 512 independent Nose-Hoover trajectories,
-N log-uniform in 500..32768 (3.95 M atoms), rho* U(0.01,0.05), T* U(0.6,0.8),
+N log-uniform in 500..32768 (4.39 M atoms), rho* U(0.01,0.05), T* U(0.6,0.8),
 cutoff 2.5 sigma, list radius 2.8 sigma, dt 0.005 tau, Stillinger analysis
 every 100 steps with first-passage time to a 50-atom cluster.  A "step" is
 one integrator step of every trajectory (rebuilds, forces, thermostat and the
@@ -16,9 +16,11 @@ Reported:
   speedup          value / fp64_plain
   accuracy         RMS relative force error of the mixed path vs FP64 on the
                    same state; energy-drift ratio mixed/FP64
-  roofline         dominant kernel: its mean duration over K more steps, run
+  roofline         dominant kernel (the force kernel or the drift, whichever
+                   is slower): its mean duration over K more steps, run
                    without the graph, from HIP events each launch records at
-                   the kernel's start and end on the engine stream
+                   the kernel's start and end on the engine stream; traffic
+                   from profiles/traffic_<kernel>_<workload>.json
   cpu_baseline     the reference Fortran (flang -O3) on the host cores
   dtype            f32+f64: pair arithmetic in FP32, state, integrator and
                    reductions in FP64 (the fp64_* fields are FP64 throughout)
