@@ -151,7 +151,8 @@ int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *at
  * cluster in series[snapshot*ntraj + traj] (may be NULL) and the first step
  * at which it reached fpt_size in fpt[traj] (-1 if never; may be NULL).
  * The analysis at step i uses the state before step i is integrated, as the
- * reference's main loop (main.f:133-164). */
+ * reference's main loop (main.f:133-164).  It keeps the histograms and the
+ * largest clusters only; fcmd_stoddard makes the per-atom arrays on demand. */
 int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series,
              int *fpt, int use_graph);
 
