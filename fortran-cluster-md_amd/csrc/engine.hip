@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,8 +21,6 @@
 using namespace fcmd;
 
 namespace {
-
-constexpr double CIKB = 1.38066e-6;  // const.inc:17
 
 long long roundup(long long v, long long m) { return (v + m - 1) / m * m; }
 
@@ -72,16 +71,42 @@ __global__ void k_init_slots(int *oid, int *inv, int *typ, int n_total, const in
     inv[i] = i;
     if (rel >= p.natom) typ[i] = -1;
 }
-// per-trajectory: record the largest cluster of this analysis and the first
-// passage to fpt_size
+// per-trajectory: record the largest cluster of this analysis (-1 for a
+// trajectory stopped by an error) and the first passage to fpt_size, at the
+// trajectory's own step number (the step about to be integrated: the
+// analysis sees the state before it, as main.f:133-164 does)
 __global__ void k_record(const ClusterScalars *cls, const TrajScalars *ts, int ntraj, int *series,
-                         int *fpt, int fpt_size, int step)
+                         int *fpt, int fpt_size)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntraj) return;
+    const bool ok = !ts[t].err;
     const int big = cls[t].cbig;
-    if (series) series[t] = big;
-    if (fpt && fpt[t] < 0 && big >= fpt_size) fpt[t] = step;
+    series[t] = ok ? big : -1;
+    if (ok && fpt[t] < 0 && big >= fpt_size) fpt[t] = ts[t].step;
+}
+
+// which trajectories the next cluster analysis takes: traj, or all (-1)
+__global__ void k_set_want(ClusterScalars *cls, int ntraj, int traj)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntraj) cls[t].want = (traj < 0 || t == traj);
+}
+
+// the error code of the lowest-numbered trajectory that has one (0: none),
+// so that a check of every trajectory costs one 4-byte copy
+__global__ __launch_bounds__(256) void k_first_err(const TrajScalars *ts, int ntraj, int *out)
+{
+    __shared__ int first;
+    if (threadIdx.x == 0) first = INT_MAX;
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntraj; t += blockDim.x)
+        if (ts[t].err) {
+            atomicMin(&first, t);
+            break;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) *out = first == INT_MAX ? 0 : ts[first].err;
 }
 
 }  // namespace
@@ -115,10 +140,20 @@ struct fcmd_system {
     double *stage = nullptr;    // natom-sized staging (device)
     int *stage_i = nullptr;
     int *dstep = nullptr;       // device int for step round trips
+    int *derr = nullptr;        // device int: k_first_err's result
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int count_nmax = 0, count_warmup = 0;  // fcmd_size_counts (0: off)
     double last_ms = 0.0;
+    // fcmd_run's largest-cluster series and first-passage times on the
+    // device, kept between runs and grown when a run needs more
+    int *dseries = nullptr, *dfpt = nullptr;
+    long long series_cap = 0;
+    // HIP events around the first NAN_TIMED analyses of the last run
+    static constexpr int NAN_TIMED = 16;
+    hipEvent_t an_ev[2 * NAN_TIMED] = {};
+    double an_ms = 0.0;
+    int an_count = 0;
 
     template <typename T>
     int alloc(T **p, long long n)
@@ -135,6 +170,10 @@ struct fcmd_system {
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         for (void *p : allocs) (void)hipFree(p);
+        if (dseries) (void)hipFree(dseries);
+        if (dfpt) (void)hipFree(dfpt);
+        for (hipEvent_t e : an_ev)
+            if (e) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -160,7 +199,7 @@ void pair_consts(const fcmd_params &p, TrajParams &t)
                     c1 = six(p.sig_ab[k]);
                     eps = p.eps_ab[k];
                 } else if (a == b) {
-                    c1 = six(p.sig[a]);  // for-a-lj-n-pbc-m.f:324, :329
+                    c1 = six(p.sig[a]);  // for-a-lj-n-pbc-m.f:62, :329
                     eps = p.eps[a];
                 } else {
                     c1 = six(0.5 * (p.sig[a] + p.sig[b]));  // :334-335 Lorentz-Berthelot
@@ -235,13 +274,46 @@ int upload_params(fcmd_system *s)
 
 int check_traj(fcmd_system *s, int traj) { return (s && traj >= 0 && traj < s->ntraj) ? FCMD_OK : FCMD_ERR_ARG; }
 
+int enqueue_step(fcmd_system *s);
+
 int sync_err(fcmd_system *s)
 {
+    hipLaunchKernelGGL(k_first_err, dim3(1), dim3(256), 0, s->stream, s->d.ts, s->ntraj, s->derr);
+    HIPCHK(hipGetLastError());
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, s->derr, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    std::vector<TrajScalars> ts(s->ntraj);
-    HIPCHK(hipMemcpy(ts.data(), s->d.ts, sizeof(TrajScalars) * s->ntraj, hipMemcpyDeviceToHost));
-    for (auto &t : ts)
-        if (t.err) return t.err;
+    return err;
+}
+
+void set_want(fcmd_system *s, int traj)
+{
+    hipLaunchKernelGGL(k_set_want, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->ntraj,
+                       traj);
+}
+
+// Capture `gsteps` steps into the run graph (replacing any).  The launch
+// shapes and tunables are baked into the graph, so it is recaptured whenever
+// they change.
+int capture_graph(fcmd_system *s, int gsteps)
+{
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+        s->graph_steps = 0;
+    }
+    hipGraph_t gr;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FCMD_OK;
+    for (int k = 0; k < gsteps && !rc; ++k) rc = enqueue_step(s);
+    HIPCHK(hipStreamEndCapture(s->stream, &gr));
+    if (rc) {
+        (void)hipGraphDestroy(gr);
+        return rc;
+    }
+    HIPCHK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(gr));
+    s->graph_steps = gsteps;
     return FCMD_OK;
 }
 
@@ -413,7 +485,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.o_hacc, hbase) A(d.hacc_stop, ntraj);
     long long maxn = 1;
     for (auto &p : s->hp) maxn = std::max<long long>(maxn, p.natom + 1);
-    A(s->stage, maxn) A(s->stage_i, maxn) A(s->dstep, 4);
+    A(s->stage, maxn) A(s->stage_i, maxn) A(s->dstep, 4) A(s->derr, 4);
 #undef A
 
     std::vector<int> bt(s->g.nblk), ct(s->g.ncblk), wsz((size_t)WL_LISTS * ntraj);
@@ -434,6 +506,11 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         delete s;
         return FCMD_ERR_HIP;
     }
+    for (hipEvent_t &e : s->an_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+            delete s;
+            return FCMD_ERR_HIP;
+        }
     if (hipMemcpy(d.blk_traj, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.cblk_traj, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.wl_size, wsz.data(), sizeof(int) * wsz.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -594,6 +671,8 @@ int fcmd_get_scalars(fcmd_system *s, int traj, fcmd_scalars *sc)
     sc->nh_iters = t.nh_iters;
     sc->err = t.err;
     sc->nlist = t.nlist;
+    sc->virial = t.virial;
+    sc->pressure = t.pressure;
     return FCMD_OK;
 }
 
@@ -637,10 +716,7 @@ int fcmd_stoddard(fcmd_system *s, int traj, int step, int *cl, int *cs, int *cc,
     const TrajParams &t = s->tp[traj];
     const bool fre = t.stoddard == ST_FRE;
     const int n = t.natom;
-    std::vector<ClusterScalars> cls(s->ntraj);
-    HIPCHK(hipMemcpy(cls.data(), s->d.cls, sizeof(ClusterScalars) * s->ntraj, hipMemcpyDeviceToHost));
-    for (int k = 0; k < s->ntraj; ++k) cls[k].want = (k == traj);
-    HIPCHK(hipMemcpy(s->d.cls, cls.data(), sizeof(ClusterScalars) * s->ntraj, hipMemcpyHostToDevice));
+    set_want(s, traj);
     finalize(s);  // cluster kinetic energies need the final velocities
     launch_cluster(s->d, s->g, fre, true, s->stream);
     HIPCHK(hipGetLastError());
@@ -748,62 +824,48 @@ int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *at
 
 int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series, int *fpt, int use_graph)
 {
-    if (!s || nsteps < 0) return FCMD_ERR_ARG;
-    const int chunk = cluster_every > 0 ? cluster_every : std::min(nsteps, 50);
-    int *dseries = nullptr, *dfpt = nullptr;
+    if (!s || nsteps < 0 || cluster_every < 0) return FCMD_ERR_ARG;
     const int nsnap = cluster_every > 0 ? (nsteps + cluster_every - 1) / cluster_every : 0;
+    std::vector<int> init(s->ntraj, -1);  // lives until the copy below has run
     if (nsnap > 0) {
-        HIPCHK(hipMalloc(&dseries, sizeof(int) * (size_t)nsnap * s->ntraj));
-        HIPCHK(hipMalloc(&dfpt, sizeof(int) * s->ntraj));
-        std::vector<int> init(s->ntraj, -1);
+        // series and first-passage buffers: kept by the system, grown here
+        const long long need = (long long)nsnap * s->ntraj;
+        if (need > s->series_cap) {
+            if (s->dseries) (void)hipFree(s->dseries);
+            s->dseries = nullptr;
+            s->series_cap = 0;
+            HIPCHK(hipMalloc(&s->dseries, sizeof(int) * (size_t)need));
+            s->series_cap = need;
+        }
+        if (!s->dfpt) HIPCHK(hipMalloc(&s->dfpt, sizeof(int) * s->ntraj));
         if (fpt) std::copy(fpt, fpt + s->ntraj, init.begin());
-        HIPCHK(hipMemcpy(dfpt, init.data(), sizeof(int) * s->ntraj, hipMemcpyHostToDevice));
-        std::vector<ClusterScalars> cls(s->ntraj);
-        HIPCHK(hipMemcpy(cls.data(), s->d.cls, sizeof(ClusterScalars) * s->ntraj, hipMemcpyDeviceToHost));
-        for (auto &c : cls) c.want = 1;
-        HIPCHK(hipMemcpy(s->d.cls, cls.data(), sizeof(ClusterScalars) * s->ntraj, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(s->dfpt, init.data(), sizeof(int) * s->ntraj, hipMemcpyHostToDevice, s->stream));
+        set_want(s, -1);
     }
-    // (re)capture a graph of `gsteps` steps when asked
-    const int gsteps = std::max(1, std::min(chunk, 10));
-    if (use_graph && (!s->graph || s->graph_steps != gsteps)) {
-        if (s->graph) {
-            (void)hipGraphExecDestroy(s->graph);
-            s->graph = nullptr;
-        }
-        hipGraph_t gr;
-        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        int rc = FCMD_OK;
-        for (int k = 0; k < gsteps && !rc; ++k) rc = enqueue_step(s);
-        HIPCHK(hipStreamEndCapture(s->stream, &gr));
-        if (rc) {
-            (void)hipGraphDestroy(gr);
-            return rc;
-        }
-        HIPCHK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(gr));
-        s->graph_steps = gsteps;
-    }
-    std::vector<int> step0(1);
-    {
-        TrajScalars t;
-        HIPCHK(hipMemcpy(&t, s->d.ts, sizeof(t), hipMemcpyDeviceToHost));
-        step0[0] = t.step;
-    }
+    // the run graph: g steps (g = 10, or the analysis interval if shorter)
+    // captured once and replayed; a graph made for a longer interval than
+    // this run's analyses allow is recaptured
+    const int gsteps = cluster_every > 0 ? std::min(cluster_every, 10) : 10;
+    if (use_graph && (!s->graph || (cluster_every > 0 && s->graph_steps > cluster_every)))
+        if (int rc = capture_graph(s, gsteps)) return rc;
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     int done = 0, snap = 0;
+    s->an_count = 0;
     while (done < nsteps) {
         if (nsnap > 0 && done % cluster_every == 0) {
+            const bool timed = snap < fcmd_system::NAN_TIMED;
+            if (timed) HIPCHK(hipEventRecord(s->an_ev[2 * snap], s->stream));
             finalize(s);
             launch_cluster(s->d, s->g, s->any_fre, false, s->stream);
             hipLaunchKernelGGL(k_record, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->d.ts,
-                               s->ntraj, dseries + (size_t)snap * s->ntraj, dfpt, fpt_size, step0[0] + done);
-            if (s->count_nmax > 0)
-                launch_size_counts(s->d, s->g, s->count_nmax, s->count_warmup, step0[0] + done, s->stream);
+                               s->ntraj, s->dseries + (size_t)snap * s->ntraj, s->dfpt, fpt_size);
+            if (s->count_nmax > 0) launch_size_counts(s->d, s->g, s->count_nmax, s->count_warmup, s->stream);
+            if (timed) HIPCHK(hipEventRecord(s->an_ev[2 * snap + 1], s->stream));
             ++snap;
         }
         const int todo = std::min(nsteps - done, nsnap > 0 ? cluster_every - done % cluster_every : nsteps - done);
         int k = 0;
-        if (use_graph)
+        if (use_graph && s->graph)
             for (; k + s->graph_steps <= todo; k += s->graph_steps) HIPCHK(hipGraphLaunch(s->graph, s->stream));
         for (; k < todo; ++k)
             if (int rc = enqueue_step(s)) return rc;
@@ -816,11 +878,17 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
+    s->an_ms = 0.0;
+    s->an_count = std::min(snap, (int)fcmd_system::NAN_TIMED);
+    for (int k = 0; k < s->an_count; ++k) {
+        float a = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, s->an_ev[2 * k], s->an_ev[2 * k + 1]));
+        s->an_ms += a;
+    }
     if (nsnap > 0) {
-        if (series) HIPCHK(hipMemcpy(series, dseries, sizeof(int) * (size_t)nsnap * s->ntraj, hipMemcpyDeviceToHost));
-        if (fpt) HIPCHK(hipMemcpy(fpt, dfpt, sizeof(int) * s->ntraj, hipMemcpyDeviceToHost));
-        (void)hipFree(dseries);
-        (void)hipFree(dfpt);
+        if (series)
+            HIPCHK(hipMemcpy(series, s->dseries, sizeof(int) * (size_t)nsnap * s->ntraj, hipMemcpyDeviceToHost));
+        if (fpt) HIPCHK(hipMemcpy(fpt, s->dfpt, sizeof(int) * s->ntraj, hipMemcpyDeviceToHost));
     }
     return sync_err(s);
 }
@@ -855,6 +923,14 @@ int fcmd_last_run_ms(fcmd_system *s, double *total_ms)
     return FCMD_OK;
 }
 
+int fcmd_last_run_analysis_ms(fcmd_system *s, double *mean_ms, int *count)
+{
+    if (!s || !mean_ms) return FCMD_ERR_ARG;
+    *mean_ms = s->an_count > 0 ? s->an_ms / s->an_count : 0.0;
+    if (count) *count = s->an_count;
+    return FCMD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Tuning.  The variants differ only in how many loads are in flight and in
 // launch widths: every one sums the same terms in the same order.
@@ -871,12 +947,12 @@ bool one_of(int v, const int *set, int n)
     return false;
 }
 
-void drop_graph(fcmd_system *s)
+// after the launch shapes change: a captured run graph is captured again
+// with the new ones (same length), so that the next run replays at once
+int recapture_graph(fcmd_system *s)
 {
-    if (s->graph) {
-        (void)hipGraphExecDestroy(s->graph);
-        s->graph = nullptr;
-    }
+    if (!s->graph) return FCMD_OK;
+    return capture_graph(s, s->graph_steps);
 }
 }  // namespace
 
@@ -888,8 +964,7 @@ int fcmd_set_tuning(fcmd_system *s, const int *tuning)
     s->g.force_u = tuning[0];
     s->g.nbr_cu = tuning[1];
     s->g.wl_grid = tuning[2];
-    drop_graph(s);
-    return FCMD_OK;
+    return recapture_graph(s);
 }
 
 int fcmd_get_tuning(fcmd_system *s, int *tuning)
@@ -948,7 +1023,7 @@ int fcmd_autotune(fcmd_system *s, int rounds, int *tuning)
     s->g.force_u = FORCE_U[bf];
     s->g.nbr_cu = NBR_CU[br / 3];
     s->g.wl_grid = WL_GRIDS[br % 3];
-    drop_graph(s);
+    if (int rc = recapture_graph(s)) return rc;
     if (tuning) fcmd_get_tuning(s, tuning);
     return sync_err(s);
 }

@@ -23,7 +23,7 @@ constexpr int BLK = 256;    // atoms per block (4 wavefronts of 64)
 constexpr int TILE = 64;    // atoms per neighbour-list tile (one wavefront)
 constexpr int MAXS = 10;    // species, const.inc:12 mxsorts
 constexpr int EMAX = 48;    // cluster edges (r <= atskin) kept per atom
-constexpr int NPART = 4;    // per-block partial sums
+constexpr int NPART = 5;    // per-block partial sums
 constexpr int NXCD = 8;     // XCDs of one MI355X, each with its own L2
 // per coordinate in a packed cell: a byte, which one v_cvt_f32_ubyte turns
 // into a float (the pair loops work with cell coordinates as floats)
@@ -52,7 +52,9 @@ enum { ST_N = 0, ST_NM = 1, ST_FRE = 2 };         // stoddard-pbc-n.f, -n-m.f, -
 enum { ERR_OK = 0, ERR_SKIN = 1, ERR_LISTFULL = 2, ERR_NH = 3, ERR_EDGES = 4, ERR_CELLS = 5 };
 
 // partial-sum slots
-enum { P_DMAX = 0, P_SUMV2 = 1, P_EPOT = 2, P_EKS = 3 };
+enum { P_DMAX = 0, P_SUMV2 = 1, P_EPOT = 2, P_EKS = 3, P_VIR = 4 };
+
+constexpr double CIKB = 1.38066e-6;  // const.inc:17, Boltzmann constant (internal units)
 
 struct TrajParams {
     double box[3], ibox[3];
@@ -109,6 +111,10 @@ struct TrajScalars {
     double dmmh[256];
     double dmms, dmmsu, dmmax, dmmav;
     double dmmsr;                    // dmms when the last rebuild was decided
+    // pair virial W = sum over pairs of r_ij . f_ij and the pressure
+    // 1e13/V (N k T - W/3) of for-a-lj-n-pbc.f:106,114-117 (commented out
+    // there), from the last force evaluation
+    double virial, pressure;
     double rs2;                      // list radius^2 of the current list
     double cside[3];                 // cell side used by the current list
     double icside[3];                // 1 / cside

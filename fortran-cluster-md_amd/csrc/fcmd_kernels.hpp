@@ -62,7 +62,8 @@ void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s);
 // per_atom: also the reference-order per-atom outputs (o_cl, o_cs, o_cek, o_cgs,
 // o_clf, o_ccf); the histograms and the largest clusters are always made
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, bool per_atom, hipStream_t s);
-// after launch_cluster: sizer-style cluster-size counts of this analysis
-void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int warmup, int step, hipStream_t s);
+// after launch_cluster: sizer-style cluster-size counts of this analysis (each
+// trajectory at its own step number)
+void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int warmup, hipStream_t s);
 
 }  // namespace fcmd

@@ -1,7 +1,7 @@
 // kernels_cluster.hip -- Stillinger and ten Wolde-Frenkel cluster analysis.
 //
 // Replaces the sequential label sweep of stoddard-pbc-n.f:69-148,
-// stoddard-pbc-n-m.f:222-307 and stoddard-pbc-n-m-fre.f:84-258.
+// stoddard-pbc-n-m.f:70-155 and stoddard-pbc-n-m-fre.f:84-258.
 //
 // Edges are neighbour-list pairs with r2 <= atskin^2, r2 formed with the
 // reference's minimum image  d + L*(bf1 - int(bf2 + d/L)),  bf1 = 2 +
@@ -383,12 +383,14 @@ __global__ __launch_bounds__(BLK) void k_cl_sums(DevPtrs d)
 // histogram, SWITCH = 1): one block per trajectory adds this analysis's
 // csi(g) to its counts for g = 1 .. the first size >= nmax present
 // (inclusive: sizer counts that row, then sets its stop flag), and stops the
-// trajectory there.  Snapshots before step `warmup` add nothing but can stop
-// the counting, as in sizer (its flag is set whatever the step).
-__global__ __launch_bounds__(BLK) void k_size_counts(DevPtrs d, int nmax, int warmup, int step)
+// trajectory there.  Snapshots before step `warmup` of the trajectory's own
+// step counter add nothing but can stop the counting, as in sizer (its flag
+// is set whatever the step).
+__global__ __launch_bounds__(BLK) void k_size_counts(DevPtrs d, int nmax, int warmup)
 {
     const int t = blockIdx.x;
     if (d.hacc_stop[t] || !d.cls[t].want || d.ts[t].err) return;
+    const int step = d.ts[t].step;
     const TrajParams &tp = d.tp[t];
     const int *csi = d.o_csi + tp.hbase;
     __shared__ int first;
@@ -406,9 +408,9 @@ __global__ __launch_bounds__(BLK) void k_size_counts(DevPtrs d, int nmax, int wa
     if (threadIdx.x == 0 && first != INT_MAX) d.hacc_stop[t] = 1;
 }
 
-void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int warmup, int step, hipStream_t s)
+void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int warmup, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_size_counts, dim3(g.ntraj), dim3(BLK), 0, s, d, nmax, warmup, step);
+    hipLaunchKernelGGL(k_size_counts, dim3(g.ntraj), dim3(BLK), 0, s, d, nmax, warmup);
 }
 
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, bool per_atom, hipStream_t s)

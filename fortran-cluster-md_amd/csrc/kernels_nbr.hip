@@ -1,7 +1,7 @@
 // kernels_nbr.hip -- cell-binned neighbour-list rebuild.
 //
 // Replaces the O(N^2) half-list build of neighbor-pbc.f:62-94 /
-// neighbor-pbc-mav.f:197-229.  The drift kernel's per-trajectory epilogue
+// neighbor-pbc-mav.f:67-99.  The drift kernel's per-trajectory epilogue
 // raises a trajectory's rebuild flag; k_wl_fill then lists the atom blocks,
 // cell chunks and trajectories of the trajectories that rebuild now, and
 // every rebuild kernel loops over those lists with a fixed grid.  A step in

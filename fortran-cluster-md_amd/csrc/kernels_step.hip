@@ -1,10 +1,10 @@
 // kernels_step.hip -- one MD step: drift, LJ forces, velocity update.
 //
 // Reference routines followed (cluster/):
-//   drift      int-verlet.f:34-41, int-verlet-mav.f:107-119, int-verlet-nh-mav.f:49-61
-//   mav/skin   int-verlet-mav.f:127-137, neighbor-pbc-mav.f:178-193, neighbor-pbc.f:40-59
-//   forces     for-a-lj-n-pbc.f:54-127, for-a-lj-n-pbc-m.f:323-414
-//   kick       int-verlet.f:53-59, int-verlet-mav.f:149-155, int-verlet-nh-mav.f:91-180
+//   drift      int-verlet.f:34-41, int-verlet-mav.f:44-56, int-verlet-nh-mav.f:49-61
+//   mav/skin   int-verlet-mav.f:64-74, neighbor-pbc-mav.f:48-63, neighbor-pbc.f:40-59
+//   forces     for-a-lj-n-pbc.f:54-127, for-a-lj-n-pbc-m.f:61-152
+//   kick       int-verlet.f:53-59, int-verlet-mav.f:86-92, int-verlet-nh-mav.f:91-180
 //
 // Element-wise updates use the reference's operand order and are compiled
 // without FMA contraction, so each per-atom update rounds as the reference
@@ -41,7 +41,7 @@ __device__ void set_cell_grid(const TrajParams &tp, TrajScalars &ts)
 }
 
 // Scalar part of neighbor(step): skin checks and adaptation, then flag the
-// list rebuild.  neighbor-pbc-mav.f:178-193 / neighbor-pbc.f:40-59, :96-100.
+// list rebuild.  neighbor-pbc-mav.f:48-63 / neighbor-pbc.f:40-59, :96-100.
 __device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *step)
 {
     const double rc = tp.atrcut;
@@ -83,20 +83,24 @@ __device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *ste
 
 // Fixed-order reduction of one trajectory's per-block partials by one block
 // (deterministic: thread k takes blocks k, k+BLK, ...; then a fixed tree).
-// Returns in thread 0: sum of slot s0, sum of slot s1, max or sum of slot s2.
+// Returns in thread 0: the sums of slots s0, s1 and s3 (s3 < 0: none) and
+// the max or sum of slot s2.
 __device__ void reduce_partials(const TrajParams &tp, const double *part, int s0, int s1, int s2,
-                                bool max2, double *lds, double &r0, double &r1, double &r2)
+                                bool max2, int s3, double *lds, double &r0, double &r1, double &r2,
+                                double &r3)
 {
-    double a = 0.0, b = 0.0, c = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
     for (int k = threadIdx.x; k < tp.nblk; k += BLK) {
         const double *p = part + (size_t)(tp.bfirst + k) * NPART;
         a += p[s0];
         b += p[s1];
         c = max2 ? fmax(c, p[s2]) : c + p[s2];
+        if (s3 >= 0) e += p[s3];
     }
     r0 = block_sum(a, lds);
     r1 = block_sum(b, lds);
     r2 = max2 ? block_max(c, lds) : block_sum(c, lds);
+    r3 = s3 >= 0 ? block_sum(e, lds) : 0.0;
 }
 
 // After the drift: thermostat half update, displacement bookkeeping and the
@@ -255,8 +259,8 @@ __global__ __launch_bounds__(BLK) void k_traj_pre(DevPtrs d)
     const TrajParams &tp = d.tp[t];
     TrajScalars &ts = d.ts[t];
     if (ts.err) return;
-    double unused, sumv2, dmmr;
-    reduce_partials(tp, d.part, P_EPOT, P_SUMV2, P_DMAX, true, lds, unused, sumv2, dmmr);
+    double unused, sumv2, dmmr, none;
+    reduce_partials(tp, d.part, P_EPOT, P_SUMV2, P_DMAX, true, -1, lds, unused, sumv2, dmmr, none);
     if (threadIdx.x == 0) traj_pre(tp, ts, dmmr, sumv2);
 }
 
@@ -308,16 +312,23 @@ __device__ void nh_solve(const TrajParams &tp, TrajScalars &ts, double S)
 // Finish of AccAtom/the integrator for one trajectory (one thread).  esum:
 // per-atom pair energies over the full list; eks: VV kinetic energy, or the
 // Nose-Hoover S; ekfree: kinetic energy of atoms not coupled to the
-// thermostat (species-selective coupling only).
+// thermostat (species-selective coupling only); wsum: per-atom pair virial
+// terms over the full list.
 __device__ void traj_post(const TrajParams &tp, TrajScalars &ts, double esum, double eks, double ekfree,
-                          int mode)
+                          double wsum, int mode)
 {
     // every pair is seen from both atoms of the full list
     esum = 0.5 * esum;
-    if (tp.accatom == ACC_1)
+    double w = 0.5 * wsum;
+    if (tp.accatom == ACC_1) {
         ts.epot = esum * tp.eps1 * 4.0;
-    else
+        w = w * 48.0 * tp.eps1;  // for-a-lj-n-pbc.f:114
+    } else {
         ts.epot = esum;
+    }
+    // for-a-lj-n-pbc.f:116-117: 1e13/V (N k T - W/3), T the target temperature
+    ts.virial = w;
+    ts.pressure = (1.0e13 / (tp.box[0] * tp.box[1] * tp.box[2])) * ((double)tp.natom * CIKB * tp.att - w / 3.0);
     if (mode == FORCE_ACCATOM) return;
     if (tp.integrator == INT_NH) {
         nh_solve(tp, ts, eks);
@@ -337,15 +348,15 @@ __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
     const TrajParams &tp = d.tp[t];
     TrajScalars &ts = d.ts[t];
     if (ts.err) return;
-    double esum, eks, ekfree;
-    reduce_partials(tp, d.part, P_EPOT, P_EKS, P_SUMV2, false, lds, esum, eks, ekfree);
-    if (threadIdx.x == 0) traj_post(tp, ts, esum, eks, ekfree, mode);
+    double esum, eks, ekfree, wsum;
+    reduce_partials(tp, d.part, P_EPOT, P_EKS, P_SUMV2, false, P_VIR, lds, esum, eks, ekfree, wsum);
+    if (threadIdx.x == 0) traj_post(tp, ts, esum, eks, ekfree, wsum, mode);
 }
 
 // ---------------------------------------------------------------------------
 // Pair sum of one atom over its full-list neighbours.  FORM = ACC_1:
 // for-a-lj-n-pbc.f:95-105 (sum of r^-2 r^-6 (c2 r^-6 - c3) d, scaled by
-// 48 eps / m afterwards); FORM = ACC_M: for-a-lj-n-pbc-m.f:377-400 (per-pair
+// 48 eps / m afterwards); FORM = ACC_M: for-a-lj-n-pbc-m.f:115-138 (per-pair
 // 48 eps_ab r^-2 r^-6 (c2_ab r^-6 - c3_ab) d).  Neighbours are taken U at a
 // time, all loads of a chunk issued before its arithmetic; pairs are summed
 // in list order.
@@ -452,7 +463,7 @@ __device__ __forceinline__ void walk_list(unsigned (*stage)[BLK], const H &h, Lo
 template <int FORM, int U, typename H>
 __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                               int i, const H &h, int ti, double &fx, double &fy, double &fz,
-                                              double &e, unsigned (*stage)[BLK])
+                                              double &e, double &w, unsigned (*stage)[BLK])
 {
     const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
@@ -490,12 +501,18 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
                 const double rij6 = rij2 * rij2 * rij2;
                 double fij;
                 if (FORM == ACC_1) {
-                    fij = rij2 * rij6 * (tp.c2[0] * rij6 - tp.c3[0]);
+                    // q: the bracket the force and the virial term
+                    // (for-a-lj-n-pbc.f:106) share
+                    const double q = tp.c2[0] * rij6 - tp.c3[0];
+                    fij = rij2 * rij6 * q;
                     e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
+                    w += rij6 * q;
                 } else {
                     const int pij = ti * MAXS + tj[u];
-                    fij = tp.e48[pij] * rij2 * rij6 * (tp.c2[pij] * rij6 - tp.c3[pij]);
+                    const double q = tp.c2[pij] * rij6 - tp.c3[pij];
+                    fij = tp.e48[pij] * rij2 * rij6 * q;
                     e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
+                    w += tp.e48[pij] * rij6 * q;
                 }
                 fx = fx + fij * dx;
                 fy = fy + fij * dy;
@@ -519,7 +536,8 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
 template <int FORM, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
-                                               double &fy, double &fz, double &e, unsigned (*stage)[BLK])
+                                               double &fy, double &fz, double &e, double &w,
+                                               unsigned (*stage)[BLK])
 {
     const float cxi = cell_xf(pi.w), cyi = cell_yf(pi.w), czi = cell_zf(pi.w);
     const float ncx = (float)ts.ncx, ncy = (float)ts.ncy, ncz = (float)ts.ncz;
@@ -566,18 +584,25 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
                 const float ri2 = __builtin_amdgcn_rcpf(r2);
                 float fij;
                 if (FORM == ACC_1) {
-                    // in sigma units, s6 = (sigma/r)^6: r^-2 r^-6 (sigma^12 r^-6 -
-                    // sigma^6/2) = r^-2 s6 (s6 - 1/2) and r^-6 (sigma^12 r^-6 -
-                    // sigma^6) = s6 (s6 - 1); the constants are immediates
+                    // in sigma units, s6 = (sigma/r)^6: the virial term
+                    // r^-6 (sigma^12 r^-6 - sigma^6/2) = s6^2 - s6/2, the force
+                    // r^-2 s6 (s6 - 1/2) = r^-2 times that, the energy term
+                    // r^-6 (sigma^12 r^-6 - sigma^6) = s6^2 - s6; the
+                    // constants are immediates
                     const float u2 = sig2 * ri2;
                     const float s6 = u2 * u2 * u2;
-                    fij = ri2 * s6 * (s6 - 0.5f);
-                    e += (double)(s6 * (s6 - 1.0f));
+                    const float s12 = s6 * s6;
+                    const float wt = __builtin_fmaf(s6, -0.5f, s12);
+                    fij = ri2 * wt;
+                    e += (double)(s12 - s6);
+                    w += (double)wt;
                 } else {
                     const float r6 = ri2 * ri2 * ri2;
                     const int pij = ti * MAXS + tj[u];
-                    fij = (float)tp.e48[pij] * ri2 * r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
+                    const float wt = r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
+                    fij = (float)tp.e48[pij] * ri2 * wt;
                     e += tp.e4[pij] * (double)(r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]));
+                    w = __builtin_fma(tp.e48[pij], (double)wt, w);
                 }
                 const double fd = (double)fij;
                 sfx = __builtin_fma((double)dxf, fd, sfx);
@@ -595,18 +620,18 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
 template <typename AccT, int FORM, int U, typename LT>
 __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                          int i, int rel, int ti, double &fx, double &fy, double &fz,
-                                         double &e, unsigned (*stage)[BLK])
+                                         double &e, double &w, unsigned (*stage)[BLK])
 {
     // the list's first loads go out before anything waits on memory
     const ListHead<LT> h(d, tp, i, rel);
     if constexpr (sizeof(AccT) == 4) {
         const float4 pi = d.posf[i];
         if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, stage);
+            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
         else
-            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, stage);
+            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
     } else
-        pair_sum_fp64<FORM, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, stage);
+        pair_sum_fp64<FORM, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, w, stage);
 }
 
 // LJ forces over the full neighbour list plus the integrator's fused
@@ -626,7 +651,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
     DevPtrs d, int mode, int part)
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
-    __shared__ double lds[3][4];
+    __shared__ double lds[4][4];
     __shared__ unsigned stage[STAGE][BLK];  // each thread's first list slots
     int b;
     if (part == FP_REBUILT) {
@@ -648,7 +673,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
     const int rel = i - tp.abase;
     const bool act = rel < tp.natom;
     const bool form1 = !MULTI || tp.accatom == ACC_1;
-    double e = 0.0, eks = 0.0, ekfree = 0.0;
+    double e = 0.0, eks = 0.0, ekfree = 0.0, w = 0.0;
     if (act) {
         const int ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
         // the velocities, loaded before the pair sum so that they arrive
@@ -661,10 +686,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
         }
         double fx, fy, fz;
         if (form1)
-            pair_sum<AccT, ACC_1, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
+            pair_sum<AccT, ACC_1, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
         else
-            pair_sum<AccT, ACC_M, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, stage);
-        // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:408-414
+            pair_sum<AccT, ACC_M, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+        // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:146-152
         double ax, ay, az;
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
         if (mode == FORCE_ACCATOM) load_acc<AccT>(d, i, ax0, ay0, az0);
@@ -717,12 +742,13 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
             }
         }
     }
-    double r[3] = {e, eks, ekfree};
-    block_reduce_end<3>(r, {false, false, false}, lds);
+    double r[4] = {e, eks, ekfree, w};
+    block_reduce_end<4>(r, {false, false, false, false}, lds);
     if (threadIdx.x == 0) {
         d.part[(size_t)b * NPART + P_EPOT] = r[0];
         d.part[(size_t)b * NPART + P_EKS] = r[1];
         d.part[(size_t)b * NPART + P_SUMV2] = r[2];
+        d.part[(size_t)b * NPART + P_VIR] = r[3];
     }
 }
 

@@ -4,10 +4,10 @@
 `System` holds one or more trajectories on the GPU and offers the reference's
 entry points with the reference's semantics:
 
-    neighbor(step)     neighbor-pbc.f:11 / neighbor-pbc-mav.f:143
-    accatom()          for-a-lj-n-pbc.f:14 / for-a-lj-n-pbc-m.f:280 (a += F/m)
-    integrator(step)   int-verlet.f:16 / int-verlet-mav.f:81 / int-verlet-nh-mav.f:13
-    stoddard(step)     stoddard-pbc-n.f:10 / -n-m.f:163 / -n-m-fre.f:11
+    neighbor(step)     neighbor-pbc.f:11 / neighbor-pbc-mav.f:13
+    accatom()          for-a-lj-n-pbc.f:14 / for-a-lj-n-pbc-m.f:18 (a += F/m)
+    integrator(step)   int-verlet.f:16 / int-verlet-mav.f:18 / int-verlet-nh-mav.f:13
+    stoddard(step)     stoddard-pbc-n.f:10 / -n-m.f:11 / -n-m-fre.f:11
 
 Errors come back as FcmdError carrying the reference's STOP message.  There is
 no CPU fallback: without the library (or a GPU) every call raises.
@@ -72,6 +72,7 @@ class Scalars(C.Structure):
         ("dmmsr", C.c_double),
         ("dmmst", C.c_int), ("dmmi", C.c_int), ("step", C.c_int), ("npairs", C.c_int),
         ("nrebuild", C.c_int), ("nh_iters", C.c_int), ("err", C.c_int), ("nlist", C.c_int),
+        ("virial", C.c_double), ("pressure", C.c_double),
     ]
 
 
@@ -87,7 +88,8 @@ EXPORTS = ("fcmd_device_count", "fcmd_error_text", "fcmd_create", "fcmd_destroy"
            "fcmd_get_scalars", "fcmd_neighbor", "fcmd_accatom", "fcmd_integrator",
            "fcmd_stoddard", "fcmd_halflist", "fcmd_run", "fcmd_init_random",
            "fcmd_last_run_ms", "fcmd_time_kernels", "fcmd_list_entries", "fcmd_set_tuning",
-           "fcmd_get_tuning", "fcmd_autotune", "fcmd_size_counts", "fcmd_get_size_counts")
+           "fcmd_get_tuning", "fcmd_autotune", "fcmd_size_counts", "fcmd_get_size_counts",
+           "fcmd_last_run_analysis_ms")
 
 
 def lib():
@@ -305,6 +307,13 @@ class System:
         v = C.c_double()
         check(lib().fcmd_last_run_ms(self.h, C.byref(v)))
         return v.value
+
+    def last_run_analysis_ms(self):
+        """(mean ms of one cluster analysis of the last run, analyses timed)."""
+        v = C.c_double()
+        n = C.c_int()
+        check(lib().fcmd_last_run_analysis_ms(self.h, C.byref(v), C.byref(n)))
+        return v.value, n.value
 
     def time_kernels(self, nsteps: int) -> dict:
         """Mean ms per step of each kernel class (HIP events, engine stream)."""

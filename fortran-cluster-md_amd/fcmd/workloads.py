@@ -140,7 +140,7 @@ def nh_initial_psi(p: Params, st: dict) -> float:
 def kinetic_energy(p: Params, st: dict) -> float:
     """ekin as the integrators define it: 0.5 m1 sum v^2 for the reference
     Nose-Hoover (int-verlet-nh-mav.f:180), per-species masses otherwise
-    (int-verlet-mav.f:154)."""
+    (int-verlet-mav.f:91)."""
     v2 = st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2
     if p.integrator == INT_NH and not p.nh_mask:
         return 0.5 * p.mas[0] * v2.sum()

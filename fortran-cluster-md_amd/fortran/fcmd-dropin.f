@@ -87,7 +87,7 @@
 	if (env(1:5).eq.'mixed') prec = FCMD_MIXED
 	env = ' '
 	call get_environment_variable('FCMD_RESIDENT', env)
-	fcmd_resident = env(1:1).eq.'1'
+	fcmd_resident = env(1:1).ne.'0'
 
 	rc = fcmd_create(fcmd_sys, 1, prm, prec, 0)
 	call fcmd_stop_on_error(rc)
@@ -195,10 +195,11 @@
 
 *-------------------------------------------------------------------------------
 * After integrator(step): the per-rebuild line the reference's neighbor
-* prints (neighbor-pbc-mav.f:232-236), then the ATOMC download if the driver
+* prints (neighbor-pbc-mav.f:102-106), then the ATOMC download if the driver
 * reads ATOMC before the next integrator call (main.f:119-162 runs its
-* writers, thermostat, resize and particle changes at step+1), or at the end
-* of the 100 warm-up steps of init-cube1.f:119-125 (stopdead follows).
+* writers, thermostat, resize and particle changes at step+1), or while
+* the driver is still initialising (fcmd_initphase: warm-up steps such as
+* init-cube1.f:119-125 are followed by host code that reads ATOMC).
 	subroutine fcmdafter(step)
 
 	use fcmd_dropin_state
@@ -233,7 +234,7 @@
 	if (ethst.eq.'X') need = .true.
 	if (bshr.eq.'X') need = .true.
 	if (chgNctl.eq.'X') need = .true.
-	if (step.eq.100) need = .true.
+	if (fcmd_initphase) need = .true.
 
 	if (need) then
 	  call fcmdpull()
@@ -247,7 +248,7 @@
 	end
 
 *-------------------------------------------------------------------------------
-* the rebuild line of neighbor-pbc.f:97-101 / neighbor-pbc-mav.f:232-236
+* the rebuild line of neighbor-pbc.f:97-101 / neighbor-pbc-mav.f:102-106
 	subroutine fcmdlog(step, sc)
 
 	use fcmd_dropin_state

@@ -30,6 +30,7 @@ module fcmd_capi
      real(c_double) :: dmmh(256)
      real(c_double) :: dmms, dmmsu, dmmax, dmmav, dmmsr
      integer(c_int) :: dmmst, dmmi, step, npairs, nrebuild, nh_iters, err, nlist
+     real(c_double) :: virial, pressure
   end type fcmd_scalars
 
   interface

@@ -3,7 +3,7 @@
 *===============================================================================
 * file: for-a-lj-n-pbc-m-hip.f
 *-------------------------------------------------------------------------------
-* Drop-in for for-a-lj-n-pbc-m.f:280-416: same subroutine name and
+* Drop-in for for-a-lj-n-pbc-m.f:18-154: same subroutine name and
 * arguments, computed on the GPU through the C ABI (include/fcmd.h).
 *-------------------------------------------------------------------------------
 

@@ -3,7 +3,7 @@
 *===============================================================================
 * file: int-verlet-mav-hip.f
 *-------------------------------------------------------------------------------
-* Drop-in for int-verlet-mav.f:81-158: same subroutine name and
+* Drop-in for int-verlet-mav.f:18-95: same subroutine name and
 * arguments, computed on the GPU through the C ABI (include/fcmd.h).
 *-------------------------------------------------------------------------------
 

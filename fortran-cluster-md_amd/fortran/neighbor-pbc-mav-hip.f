@@ -3,7 +3,7 @@
 *===============================================================================
 * file: neighbor-pbc-mav-hip.f
 *-------------------------------------------------------------------------------
-* Drop-in for neighbor-pbc-mav.f:143-239: same subroutine name and
+* Drop-in for neighbor-pbc-mav.f:13-109: same subroutine name and
 * arguments, computed on the GPU through the C ABI (include/fcmd.h).
 *-------------------------------------------------------------------------------
 

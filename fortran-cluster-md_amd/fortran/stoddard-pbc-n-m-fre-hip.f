@@ -21,6 +21,8 @@
 
 	call fcmdensure()
 	call fcmdpushdirty()
+*** main.f:112 calls stoddard(-1) once initialisation is over
+	if (step.lt.0) fcmd_initphase = .false.
 	cnt(1) = cnum
 	cnt(2) = cnumf
 	cnt(3) = cbiggest

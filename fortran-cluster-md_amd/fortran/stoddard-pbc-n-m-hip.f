@@ -3,7 +3,7 @@
 *===============================================================================
 * file: stoddard-pbc-n-m-hip.f
 *-------------------------------------------------------------------------------
-* Drop-in for stoddard-pbc-n-m.f:163-310: same subroutine name and
+* Drop-in for stoddard-pbc-n-m.f:11-158: same subroutine name and
 * arguments, computed on the GPU through the C ABI (include/fcmd.h).
 *-------------------------------------------------------------------------------
 
@@ -21,6 +21,8 @@
 
 	call fcmdensure()
 	call fcmdpushdirty()
+*** main.f:112 calls stoddard(-1) once initialisation is over
+	if (step.lt.0) fcmd_initphase = .false.
 	cnt(1) = cnum
 	cnt(2) = cnumf
 	cnt(3) = cbiggest

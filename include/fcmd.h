@@ -5,11 +5,11 @@
  * fortran-cluster-md, cluster/), which are external F77 subroutines linked
  * by name and sharing state through COMMON blocks (Makefile:8-114):
  *
- *   AccAtom()          for-a-lj-n-pbc.f:14, for-a-lj-n-pbc-m.f:280  -> fcmd_accatom
- *   neighbor(step)     neighbor-pbc.f:11, neighbor-pbc-mav.f:143    -> fcmd_neighbor
- *   integrator(step)   int-verlet.f:16, int-verlet-mav.f:81,
+ *   AccAtom()          for-a-lj-n-pbc.f:14, for-a-lj-n-pbc-m.f:18  -> fcmd_accatom
+ *   neighbor(step)     neighbor-pbc.f:11, neighbor-pbc-mav.f:13    -> fcmd_neighbor
+ *   integrator(step)   int-verlet.f:16, int-verlet-mav.f:18,
  *                      int-verlet-nh-mav.f:13                       -> fcmd_integrator
- *   stoddard(step)     stoddard-pbc-n.f:10, stoddard-pbc-n-m.f:163,
+ *   stoddard(step)     stoddard-pbc-n.f:10, stoddard-pbc-n-m.f:11,
  *                      stoddard-pbc-n-m-fre.f:11                    -> fcmd_stoddard
  *
  * The Fortran shims in fortran-cluster-md_amd/fortran/ keep those names and
@@ -72,14 +72,21 @@ typedef struct {
 
 /* Scalars of one trajectory (ATOMP atrskin, psi, s; ENERG ekin, epot,
  * sumv2; DMMPP), plus engine counters: dmmsr is dmms as it stood when the
- * last rebuild was decided (what neighbor-pbc-mav.f:232 prints), npairs the
+ * last rebuild was decided (what neighbor-pbc-mav.f:102 prints), npairs the
  * pairs in the current list, nrebuild the rebuilds so far, nlist the largest
- * neighbour count of any atom in the current list (at most list_width). */
+ * neighbour count of any atom in the current list (at most list_width).
+ * virial and pressure (read only) come from the last force evaluation: the
+ * pair virial W = sum over pairs of r_ij . f_ij (1e-17 J) and the pressure
+ * 1e13/V (N k T - W/3) in Pa, T the target temperature att, as the
+ * reference's commented-out lines for-a-lj-n-pbc.f:106,114-117 define them;
+ * the two-species forms sum 48 eps_ab r^-6 (sigma_ab^12 r^-6 - sigma_ab^6/2)
+ * per pair. */
 typedef struct {
     double atrskin, psi, s, ekin, epot, sumv2;
     double dmmh[256];
     double dmms, dmmsu, dmmax, dmmav, dmmsr;
     int dmmst, dmmi, step, npairs, nrebuild, nh_iters, err, nlist;
+    double virial, pressure;
 } fcmd_scalars;
 
 int fcmd_device_count(void);
@@ -114,21 +121,21 @@ int fcmd_download(fcmd_system *s, int traj, double *x, double *y, double *z, dou
 int fcmd_set_scalars(fcmd_system *s, int traj, const fcmd_scalars *sc);
 int fcmd_get_scalars(fcmd_system *s, int traj, fcmd_scalars *sc);
 
-/* neighbor(step): neighbor-pbc.f:11 / neighbor-pbc-mav.f:143.  Wraps the
+/* neighbor(step): neighbor-pbc.f:11 / neighbor-pbc-mav.f:13.  Wraps the
  * positions, applies the skin rules and rebuilds the list of trajectory traj;
  * *step is made non-negative as the reference does. */
 int fcmd_neighbor(fcmd_system *s, int traj, int *step);
 
-/* AccAtom(): for-a-lj-n-pbc.f:14 / for-a-lj-n-pbc-m.f:280.  Adds the LJ
+/* AccAtom(): for-a-lj-n-pbc.f:14 / for-a-lj-n-pbc-m.f:18.  Adds the LJ
  * accelerations to a and sets epot, for every trajectory. */
 int fcmd_accatom(fcmd_system *s);
 
 /* integrator(step): one step of every trajectory (int-verlet.f:16,
- * int-verlet-mav.f:81, int-verlet-nh-mav.f:13), including the list rebuild
+ * int-verlet-mav.f:18, int-verlet-nh-mav.f:13), including the list rebuild
  * the displacement rule asks for.  *step is this step's number. */
 int fcmd_integrator(fcmd_system *s, int *step);
 
-/* stoddard(step): stoddard-pbc-n.f:10 / -n-m.f:163 / -n-m-fre.f:11 for
+/* stoddard(step): stoddard-pbc-n.f:10 / -n-m.f:11 / -n-m-fre.f:11 for
  * trajectory traj.  Arrays in the CVTPP layout; histogram arrays hold
  * natom+1 entries starting at index 0.  csi[0] and ceks[0] are updated the
  * way the reference leaves them (csi(0) is never reset there).  counters =
@@ -179,6 +186,13 @@ int fcmd_get_size_counts(fcmd_system *s, int traj, int *counts, int *stopped);
 
 /* Time of the last fcmd_run in milliseconds (HIP events on the run stream). */
 int fcmd_last_run_ms(fcmd_system *s, double *total_ms);
+
+/* Mean time in milliseconds of one cluster analysis of the last fcmd_run
+ * (the pending Nose-Hoover update, the Stillinger analysis of every
+ * trajectory and the recording of the series), from HIP events around each
+ * of its first 16 analyses; *count (may be NULL) receives how many were
+ * timed.  No counterpart in the reference (it has no timers). */
+int fcmd_last_run_analysis_ms(fcmd_system *s, double *mean_ms, int *count);
 
 /* Run nsteps steps (no graph) and time them with HIP events on the engine
  * stream; avg_ms[FCMD_NTIMED] receives the mean milliseconds per step of

@@ -8,9 +8,10 @@
  * Arrays are 0-based here; atom numbers stored in the lists are 1-based, as in
  * the reference.  Helper macros X(i) etc. take the 1-based atom number.
  *
- * Pinned against the compiled reference (oracle/_ref harness) by
- * tests/test_oracle_vs_reference.py: lists, positions, velocities, energies,
- * mav bookkeeping, labels and histograms bit-exact on the same inputs.
+ * Pinned against the compiled reference (oracle/_ref harness) by the golden
+ * replays and test_oracle_vs_compiled_reference in tests/test_oracle.py:
+ * lists, positions, velocities, energies, mav bookkeeping, labels and
+ * histograms bit-exact on the same inputs.
  */
 #include "fcmd_oracle.h"
 
@@ -41,7 +42,7 @@ static inline double fmaxd(double p, double q) { return p > q ? p : q; }
 static inline double fmind(double p, double q) { return p < q ? p : q; }
 
 /* ------------------------------------------------------------------------- */
-/* list build shared by neighbor-pbc.f:62-94 and neighbor-pbc-mav.f:197-229   */
+/* list build shared by neighbor-pbc.f:62-94 and neighbor-pbc-mav.f:67-99   */
 static int build_halflist(orc_state *st, orc_arrays *a)
 {
     const int natom = st->natom;
@@ -96,7 +97,7 @@ int orc_neighbor(int variant, orc_state *st, orc_arrays *a, int *step)
             st->atrskin = st->atrskin - 0.01 * (st->atrskin - rc);
         }
     } else {
-        /* neighbor-pbc-mav.f:178-193 */
+        /* neighbor-pbc-mav.f:48-63 */
         if (st->dmms > (st->atrskin - rc)) return ORC_ERR_SKIN;
         st->dmmav = (double)(float)st->dmmst * st->dmmsu / 256.0;
         if (*step != st->btim && *step > 0) {
@@ -109,7 +110,7 @@ int orc_neighbor(int variant, orc_state *st, orc_arrays *a, int *step)
     }
     int rc_ = build_halflist(st, a);
     if (rc_ != ORC_OK) return rc_;
-    /* neighbor-pbc.f:96-100 / neighbor-pbc-mav.f:231-235 */
+    /* neighbor-pbc.f:96-100 / neighbor-pbc-mav.f:101-105 */
     st->dmmax = fmaxd(st->dmms, st->dmmax);
     st->dmms = 0.0;
     st->dmmst = 0;
@@ -117,7 +118,7 @@ int orc_neighbor(int variant, orc_state *st, orc_arrays *a, int *step)
 }
 
 /* ------------------------------------------------------------------------- */
-/* for-a-lj-n-pbc.f:54-127 (one species) and for-a-lj-n-pbc-m.f:323-414      */
+/* for-a-lj-n-pbc.f:54-127 (one species) and for-a-lj-n-pbc-m.f:61-152      */
 int orc_accatom(int variant, orc_state *st, orc_arrays *a)
 {
     const int natom = st->natom;
@@ -128,6 +129,7 @@ int orc_accatom(int variant, orc_state *st, orc_arrays *a)
     const double bf2 = bf1 + 0.5;
     const double ibx = 1.0 / boxx, iby = 1.0 / boxy, ibz = 1.0 / boxz;
     double esum = 0.0;
+    double virial = 0.0;
     double *fx = calloc((size_t)natom, sizeof(double));
     double *fy = calloc((size_t)natom, sizeof(double));
     double *fz = calloc((size_t)natom, sizeof(double));
@@ -179,18 +181,33 @@ int orc_accatom(int variant, orc_state *st, orc_arrays *a)
                 if (variant == ORC_ACC_1) {
                     fij = rij2 * rij6 * (c2 * rij6 - c3);
                     esum = esum + rij6 * (c2 * rij6 - c1);
-                } else {
+                    virial = virial + rij6 * (c2 * rij6 - c3); /* :106 */
+                } else if (variant == ORC_ACC_M) {
                     const int ij = API(i) + API(j);
                     if (ij == 2) {
                         fij = eps11 * 48.0 * rij2 * rij6 * (c112 * rij6 - c113);
                         esum = esum + eps11 * 4.0 * rij6 * (c112 * rij6 - c111);
+                        virial = virial + eps11 * 48.0 * rij6 * (c112 * rij6 - c113);
                     } else if (ij == 4) {
                         fij = eps22 * 48.0 * rij2 * rij6 * (c222 * rij6 - c223);
                         esum = esum + eps22 * 4.0 * rij6 * (c222 * rij6 - c221);
+                        virial = virial + eps22 * 48.0 * rij6 * (c222 * rij6 - c223);
                     } else {
                         fij = eps12 * 48.0 * rij2 * rij6 * (c122 * rij6 - c123);
                         esum = esum + eps12 * 4.0 * rij6 * (c122 * rij6 - c121);
+                        virial = virial + eps12 * 48.0 * rij6 * (c122 * rij6 - c123);
                     }
+                } else {
+                    /* explicit pair table: the -m expressions with the
+                     * pair's own eps and sigma */
+                    const int ab = (API(i) - 1) * ORC_MAXSORTS + (API(j) - 1);
+                    const double sg = st->sig_ab[ab], ep = st->eps_ab[ab];
+                    const double t1 = sg * sg * sg * sg * sg * sg;
+                    const double t2 = t1 * t1;
+                    const double t3 = 0.5 * t1;
+                    fij = ep * 48.0 * rij2 * rij6 * (t2 * rij6 - t3);
+                    esum = esum + ep * 4.0 * rij6 * (t2 * rij6 - t1);
+                    virial = virial + ep * 48.0 * rij6 * (t2 * rij6 - t3);
                 }
                 fx[i - 1] = fx[i - 1] + fij * dx;
                 fx[j - 1] = fx[j - 1] - fij * dx;
@@ -202,6 +219,10 @@ int orc_accatom(int variant, orc_state *st, orc_arrays *a)
         }
     }
 
+    /* for-a-lj-n-pbc.f:114-117 (the -m and table forms fold 48 eps per pair) */
+    if (variant == ORC_ACC_1) virial = virial * 48.0 * st->eps[0];
+    st->virial = virial;
+    st->pressure = (1.0e13 / (boxx * boxy * boxz)) * ((double)natom * CIKB * st->att - virial / 3.0);
     if (variant == ORC_ACC_1) {
         const double ate = st->eps[0];
         const double iatm = 1.0 / st->mas[0];
@@ -238,7 +259,7 @@ static void zero_acc(orc_state *st, orc_arrays *a)
 }
 
 /* moving-average displacement bookkeeping and rebuild trigger:
- * int-verlet-mav.f:127-137 == int-verlet-nh-mav.f:70-78 */
+ * int-verlet-mav.f:64-74 == int-verlet-nh-mav.f:70-78 */
 static int mav_trigger(orc_state *st, double dmmr)
 {
     dmmr = 2.2 * sqrt(dmmr);
@@ -281,7 +302,7 @@ int orc_integrator(int ivar, int nbvar, int accvar, orc_state *st,
     }
 
     if (ivar == ORC_INT_MAV) {
-        /* int-verlet-mav.f:107-155 */
+        /* int-verlet-mav.f:44-92 */
         double dmmr = 0.0;
         for (int i = 1; i <= natom; i++) {
             const double dx = (VX(i) + 0.5 * AX(i) * dt) * dt;

@@ -6,9 +6,10 @@
  *
  * Every routine follows one reference file, operation by operation, in FP64
  * with no FMA contraction, so that on the same inputs it reproduces the
- * reference bit for bit (pinned against the compiled reference in
- * tests/test_oracle_vs_reference.py).  Atom indices in the neighbour list are
- * 1-based and headers are negative, exactly as in COMMON /ATOMP/ atnlist.
+ * reference bit for bit (pinned against the compiled reference by the golden
+ * replays and test_oracle_vs_compiled_reference in tests/test_oracle.py).
+ * Atom indices in the neighbour list are 1-based and headers are negative,
+ * exactly as in COMMON /ATOMP/ atnlist.
  */
 #ifndef FCMD_ORACLE_H
 #define FCMD_ORACLE_H
@@ -33,6 +34,11 @@ typedef struct {
     int cnum, cnumf, cbiggest, cbiggestf, clookfor, clookforf, cstep;
     int nh_iters;                               /* Newton iterations used */
     int npairs;                                 /* pairs in the last list */
+    /* pair virial and pressure of the last AccAtom (for-a-lj-n-pbc.f:106,
+     * 114-117, commented out in the reference) */
+    double virial, pressure;
+    /* ORC_ACC_TABLE only: eps and sigma of each species pair, a*10+b */
+    double eps_ab[ORC_MAXSORTS * ORC_MAXSORTS], sig_ab[ORC_MAXSORTS * ORC_MAXSORTS];
 } orc_state;
 
 /* per-atom arrays; sizes are natom unless marked [0..natom] */
@@ -51,7 +57,10 @@ typedef struct {
 
 enum { ORC_OK = 0, ORC_ERR_SKIN = 1, ORC_ERR_LISTFULL = 2, ORC_ERR_NH = 3 };
 enum { ORC_NB_PBC = 0, ORC_NB_MAV = 1 };                  /* neighbor */
-enum { ORC_ACC_1 = 0, ORC_ACC_M = 1 };                    /* AccAtom */
+/* AccAtom: for-a-lj-n-pbc.f, for-a-lj-n-pbc-m.f, and ORC_ACC_TABLE, the
+ * -m form with eps and sigma of every species pair from eps_ab/sig_ab instead
+ * of Lorentz-Berthelot (an extension: the reference has no such file) */
+enum { ORC_ACC_1 = 0, ORC_ACC_M = 1, ORC_ACC_TABLE = 2 };
 enum { ORC_INT_VV = 0, ORC_INT_MAV = 1, ORC_INT_NH = 2 }; /* integrator */
 enum { ORC_ST_N = 0, ORC_ST_NM = 1, ORC_ST_FRE = 2 };     /* stoddard */
 

@@ -29,7 +29,7 @@ CIKB = 1.38066e-6  # const.inc:17
 CIMU = 1.66056     # const.inc:18
 
 NB_PBC, NB_MAV = 0, 1
-ACC_1, ACC_M = 0, 1
+ACC_1, ACC_M, ACC_TABLE = 0, 1, 2
 INT_VV, INT_MAV, INT_NH = 0, 1, 2
 ST_N, ST_NM, ST_FRE = 0, 1, 2
 
@@ -67,6 +67,8 @@ class OrcState(C.Structure):
         ("cnum", C.c_int), ("cnumf", C.c_int), ("cbiggest", C.c_int),
         ("cbiggestf", C.c_int), ("clookfor", C.c_int), ("clookforf", C.c_int),
         ("cstep", C.c_int), ("nh_iters", C.c_int), ("npairs", C.c_int),
+        ("virial", C.c_double), ("pressure", C.c_double),
+        ("eps_ab", C.c_double * 100), ("sig_ab", C.c_double * 100),
     ]
 
 
@@ -106,12 +108,23 @@ def lib():
 class Oracle:
     """A system held by the C restatement; mirrors the reference COMMON state."""
 
-    def __init__(self, case: dict, variant: str = "mav", list_cap: int | None = None):
+    def __init__(self, case: dict, variant: str = "mav", list_cap: int | None = None,
+                 eps_ab=None, sig_ab=None):
+        """eps_ab, sig_ab (nsorts x nsorts): the explicit pair-table form of
+        AccAtom (ORC_ACC_TABLE) instead of the variant's."""
         self.variant = variant
         self.ivar, self.nbvar, self.accvar, self.stvar = VARIANTS[variant]
         n = int(case["natom"])
         self.n = n
         st = OrcState()
+        if eps_ab is not None:
+            self.accvar = ACC_TABLE
+            ea = np.asarray(eps_ab, dtype=np.float64)
+            sa = np.asarray(sig_ab, dtype=np.float64)
+            for a in range(ea.shape[0]):
+                for b in range(ea.shape[1]):
+                    st.eps_ab[a * 10 + b] = ea[a, b]
+                    st.sig_ab[a * 10 + b] = sa[a, b]
         st.natom = n
         st.nsorts = int(case["nsorts"])
         for k in range(3):

@@ -7,7 +7,7 @@
 * and writes golden vectors for the C restatement and the HIP kernels.  It is
 * never part of the product.
 *
-* Input (stream, little endian, written by oracle/refrun.py):
+* Input (stream, little endian, written by oracle/pyoracle.py _write_input):
 *   i4 natom, atsorts, nact
 *   i4 api(natom)
 *   f8 boxx, boxy, boxz
@@ -22,7 +22,8 @@
 *     code 2: a=0, epot=0, AccAtom()    (for-a-lj-n-pbc*.f)
 *     code 3: integrator(i), i=arg1..arg2 (int-verlet*.f)
 *     code 4: stoddard(arg1)            (stoddard-pbc-n*.f)
-* Output: one record per action (see dumpstate below).
+* Output: one record per action (see dumpstate below); on stdout, after
+* each action, a line 'HARNESS-TIME code seconds' with its wall time.
 *===============================================================================
 
 	program refharness
@@ -41,6 +42,7 @@
 	character*256 fin, fout
 	integer nact, ia, code, arg1, arg2, i, ieth, istep
 	integer iacts(3,1000)
+	integer*8 ct0, ct1, crate
 
 	call getarg(1, fin)
 	call getarg(2, fout)
@@ -99,6 +101,7 @@
 	  arg1 = iacts(2,ia)
 	  arg2 = iacts(3,ia)
 	  istep = arg1
+	  call system_clock(ct0, crate)
 	  if (code.eq.1) then
 	    call neighbor(istep)
 	  elseif (code.eq.2) then
@@ -117,6 +120,9 @@
 	  elseif (code.eq.4) then
 	    call stoddard(istep)
 	  endif
+	  call system_clock(ct1)
+	  write (*,'(A,I3,1X,F16.6)') 'HARNESS-TIME', code,
+     $	    dble(ct1-ct0)/dble(crate)
 	  call dumpstate(12, code, istep)
 	enddo
 	close (12)

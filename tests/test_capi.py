@@ -35,8 +35,8 @@ def test_fortran_interface_declares_the_entry_points():
 
 def test_error_texts_are_the_reference_messages():
     t = lambda c: fcmd.lib().fcmd_error_text(c).decode()  # noqa: E731
-    assert t(1) == "*** FATAL ERROR! ***  Skin radius exceeded!!"        # neighbor-pbc-mav.f:179
-    assert t(2) == "*** FATAL ERROR *** Neighborlist is nearly full!"    # neighbor-pbc-mav.f:225
+    assert t(1) == "*** FATAL ERROR! ***  Skin radius exceeded!!"        # neighbor-pbc-mav.f:49
+    assert t(2) == "*** FATAL ERROR *** Neighborlist is nearly full!"    # neighbor-pbc-mav.f:95
     assert t(3) == "Numerical integration did not converge!"             # int-verlet-nh-mav.f:151
 
 
@@ -44,7 +44,9 @@ def test_struct_layouts_match_header():
     import ctypes as C
     # fcmd_params: 2 ints, 3+30 doubles, 10 ints, 6 doubles, 8 ints, 200 doubles, 2 ints
     assert C.sizeof(fcmd.Params) == 8 + 24 + 240 + 40 + 48 + 32 + 1600 + 8
-    assert C.sizeof(fcmd.Scalars) == 6 * 8 + 256 * 8 + 5 * 8 + 8 * 4
+    # fcmd_scalars: 6 + 256 + 5 doubles, 8 ints, 2 doubles (virial, pressure)
+    assert C.sizeof(fcmd.Scalars) == 6 * 8 + 256 * 8 + 5 * 8 + 8 * 4 + 2 * 8
+    assert fcmd.Scalars.virial.offset == 6 * 8 + 256 * 8 + 5 * 8 + 8 * 4
 
 
 @pytest.mark.skipif(fcmd.device_count() > 0, reason="checks the no-device path")

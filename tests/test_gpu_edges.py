@@ -3,8 +3,10 @@ neighbours, and the reference's fatal conditions forced from the host.
 
 * Dimer (for-a-lj-n-pbc.f:95-110): V(r) = 4 eps ((s/r)^12 - (s/r)^6) and
   a = 48 eps / m * r^-2 (s^12 r^-12 - s^6 r^-6 / 2) * dx, so V(2^(1/6) s) = -eps
-  with no force and V(s) = 0 with |F| = 24 eps / s, both on the FP64 and the
-  mixed path, on grids of 1 and 5 cells a side and across the periodic face.
+  with no force and V(s) = 0 with |F| = 24 eps / s, and the pair virial
+  W = r F(r) = 48 eps ((s/r)^12 - (s/r)^6/2) with the pressure
+  1e13/V (N k T - W/3) (:106,114-117), both on the FP64 and the mixed path,
+  on grids of 1 and 5 cells a side and across the periodic face.
 * No neighbours: forces and energy exactly zero.
 * neighbor-pbc*.f "Skin radius exceeded!!" when the largest displacement
   since the last build exceeds the skin; the Nose-Hoover update reports
@@ -43,13 +45,16 @@ def dimer_case(box_sig, r, across):
     return c
 
 
-def forces(case, precision):
+def forces(case, precision, with_virial=False):
     s = gpu_system(case, "mav", precision=precision)
     s.neighbor(0)
     n = case["natom"]
     s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
     s.accatom()
-    return s.download(), s.scalars().epot
+    sc = s.scalars()
+    if with_virial:
+        return s.download(), sc.epot, sc.virial, sc.pressure
+    return s.download(), sc.epot
 
 
 @pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
@@ -61,7 +66,7 @@ def test_lj_dimer_closed_forms(precision, box_sig, across):
     scale = 24.0 * eps / (m * SIG)  # |a| at r = sigma
     for r in (R_MIN, SIG):
         c = dimer_case(box_sig, r, across)
-        got, epot = forces(c, precision)
+        got, epot, vir, press = forces(c, precision, with_virial=True)
         L = c["box"][0]
         dx = c["x"][0] - c["x"][1]
         dx -= L * np.rint(dx / L)  # minimum image: -r, or +r across the face
@@ -78,6 +83,12 @@ def test_lj_dimer_closed_forms(precision, box_sig, across):
         else:
             assert v_ref == pytest.approx(-eps, rel=1e-6)
         assert abs(epot - v_ref) <= tol * eps
+        # pair virial W = r . f (for-a-lj-n-pbc.f:106,114): 48 eps (s6^2 - s6/2)
+        w_ref = 48.0 * eps * (sr6 * sr6 - 0.5 * sr6)
+        assert abs(vir - w_ref) <= tol * 24.0 * eps
+        V = L ** 3
+        p_ref = 1e13 / V * (2 * cases.CIKB * c["att"] - w_ref / 3)
+        assert abs(press - p_ref) <= tol * 1e13 / V * 8.0 * eps + 1e-12 * abs(p_ref)
         assert abs(got["ax"][0] - ax_ref) <= tol * scale
         assert abs(got["ax"][1] + ax_ref) <= tol * scale  # Newton's third law
         assert abs(got["az"][0]) <= tol * scale and abs(got["az"][1]) <= tol * scale

@@ -89,7 +89,7 @@ def test_golden_trajectory(name):
     cluster analysis is bit-exact against the oracle on the GPU's snapshot."""
     variant, case, actions, recs = load_golden(name)
     s = gpu_system(case, variant)
-    stt = {}
+    stt, ost = {}, {}
     for (code, a1, a2), r in zip(actions, recs):
         if code == 1:
             s.neighbor(a1)
@@ -116,21 +116,46 @@ def test_golden_trajectory(name):
                 assert sc.s == pytest.approx(r["s"], rel=1e-8, abs=1e-12)
         elif code == 4:
             o = oracle_from_gpu(s, case, variant)
+            # the first-passage bookkeeping carries over between calls, as
+            # COMMON /CVTPP/ carries it (stoddard-pbc-n-m-fre.f:229-258)
+            for k in ("cbiggest", "cbiggestf", "clookfor", "clookforf"):
+                if k in ost:
+                    setattr(o.st, k, ost[k])
+            for k in ("cfpt", "cfptf"):
+                if k in ost:
+                    o.arr[k][:] = ost[k]
             o.stoddard(a1)
+            ost.update({k: getattr(o.st, k) for k in ("cbiggest", "cbiggestf", "clookfor", "clookforf")})
+            ost.update(cfpt=o.arr["cfpt"].copy(), cfptf=o.arr["cfptf"].copy())
             out = s.stoddard(a1, state=stt)
             for k in ("cl", "cs", "csi"):
                 got = out[k] if k != "csi" else out[k][1:]
                 ref = o.arr[k] if k != "csi" else o.arr[k][1:]
                 np.testing.assert_array_equal(got, ref, err_msg=k)
             assert rel_err(out["cek"], o.arr["cek"]) < REL
+            assert out["cnum"] == o.st.cnum
             if variant == "fre":
-                for k in ("clf", "ccf", "csf", "csif", "cgs"):
-                    np.testing.assert_array_equal(out[k], o.arr[k], err_msg=k)
-                assert out["cnumf"] == o.st.cnumf
-                assert rel_err(out["cekf"], o.arr["cekf"]) < REL
+                check_twf(out, o)
             # and the fixture's own labels where the snapshot is the reference's
             if r["step"] == 0 and code == 4:
                 np.testing.assert_array_equal(out["cl"], r["cl"])
+
+
+TWF_INT = ("clf", "ccf", "csf", "csif", "cgs", "cgsf", "cgsfa", "cfpt", "cfptf")
+TWF_COUNTERS = ("cnumf", "cbiggest", "cbiggestf", "clookfor", "clookforf")
+
+
+def check_twf(out, o):
+    """Every ten Wolde-Frenkel output of stoddard-pbc-n-m-fre.f against the
+    oracle: labels and sizes (:198-225), the carrier-gas columns cgsf/cgsfa
+    and ceksf that wrt-csi-fre.f:40-55 prints, the first-passage arrays and
+    counters (:229-258); kinetic-energy sums to summation order."""
+    for k in TWF_INT:
+        np.testing.assert_array_equal(out[k], o.arr[k], err_msg=k)
+    for k in TWF_COUNTERS:
+        assert out[k] == getattr(o.st, k), k
+    assert rel_err(out["cekf"], o.arr["cekf"]) < REL
+    assert rel_err(out["ceksf"], o.arr["ceksf"]) < REL
 
 
 @pytest.mark.parametrize("n,rho,seed,variant", [
@@ -154,7 +179,10 @@ def test_larger_system_one_step(n, rho, seed, variant):
     s.accatom()
     got = s.download()
     assert acc_err(got, o.arr) < REL
-    assert abs(s.scalars().epot - o.st.epot) <= REL * abs(o.st.epot)
+    sc = s.scalars()
+    assert abs(sc.epot - o.st.epot) <= REL * abs(o.st.epot)
+    assert abs(sc.virial - o.st.virial) <= REL * abs(o.st.virial)
+    assert abs(sc.pressure - o.st.pressure) <= REL * abs(o.st.pressure)
     # one integrator step from the identical state
     o.integrator(1)
     s.integrator(1)
@@ -171,6 +199,8 @@ def test_larger_system_one_step(n, rho, seed, variant):
     out = s.stoddard(1)
     np.testing.assert_array_equal(out["cl"], o2.arr["cl"])
     np.testing.assert_array_equal(out["csi"][1:], o2.arr["csi"][1:])
+    if variant == "fre":
+        check_twf(out, o2)
 
 
 def test_list_width_is_the_widest_atom():
@@ -239,7 +269,9 @@ def test_mixed_precision_force_error():
     num = sum(((got[k] - o.arr[k]) ** 2).sum() for k in ("ax", "ay", "az"))
     den = sum((o.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
     assert np.sqrt(num / den) < 1e-5
-    assert abs(s.scalars().epot - o.st.epot) < 1e-5 * abs(o.st.epot)
+    sc = s.scalars()
+    assert abs(sc.epot - o.st.epot) < 1e-5 * abs(o.st.epot)
+    assert abs(sc.virial - o.st.virial) < 1e-5 * abs(o.st.virial)
 
 
 @pytest.mark.parametrize("n,rho", [(8, 0.05), (12, 0.05), (100, 0.05)])

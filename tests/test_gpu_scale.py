@@ -16,6 +16,7 @@ import cases
 import fcmd
 import pyoracle as po
 from fcmd import workloads as W
+from test_gpu_parity import check_twf
 
 pytestmark = pytest.mark.gpu
 
@@ -68,7 +69,7 @@ def _sample_completeness(case, lst, idx, nsample, rng):
 
 
 @pytest.mark.parametrize("workload,prec", [("lj_nh_262k", fcmd.FP64), ("lj_nh_262k", fcmd.MIXED),
-                                           ("lj_nh_2m", fcmd.FP64)])
+                                           ("lj_nh_2m", fcmd.FP64), ("lj_nh_2m", fcmd.MIXED)])
 def test_full_size_forces_lists_clusters(workload, prec):
     params, states, meta = W.WORKLOADS[workload]()
     p, st = params[0], states[0]
@@ -77,7 +78,9 @@ def test_full_size_forces_lists_clusters(workload, prec):
         W.setup(s, params, states)
         s.run(30, use_graph=True)
         got = s.download()
-        variant = "nuc"  # for-a-lj-n-pbc-m form (single species), Stillinger on the list
+        # for-a-lj-n-pbc-m form (single species); the workload's Stillinger
+        # form (stoddard-pbc-n-m.f, or -n-m-fre.f with ten Wolde-Frenkel)
+        variant = "fre" if p.stoddard == fcmd.ST_FRE else "nuc"
         o, lst, idx = _oracle_on_snapshot(s, case, variant)
         # list: sorted, unique, complete for a sample
         _sample_completeness(dict(case, **got), lst, idx, 200, np.random.default_rng(1))
@@ -87,7 +90,9 @@ def test_full_size_forces_lists_clusters(workload, prec):
         den = sum((o.arr[k] ** 2).sum() for k in ("ax", "ay", "az"))
         tol = 1e-12 if prec == fcmd.FP64 else 1e-5
         assert np.sqrt(num / den) < tol
-        assert abs(s.scalars().epot - o.st.epot) < tol * abs(o.st.epot) * 10
+        sc = s.scalars()
+        assert abs(sc.epot - o.st.epot) < tol * abs(o.st.epot) * 10
+        assert abs(sc.virial - o.st.virial) < tol * abs(o.st.virial) * 10
         # Newton's third law
         for k in ("ax", "ay", "az"):
             assert abs(got[k].sum()) < 1e-9 * np.abs(got[k]).sum()
@@ -96,6 +101,8 @@ def test_full_size_forces_lists_clusters(workload, prec):
         o.stoddard(30)
         np.testing.assert_array_equal(out["cl"], o.arr["cl"])
         np.testing.assert_array_equal(out["csi"][1:], o.arr["csi"][1:])
+        if variant == "fre":
+            check_twf(out, o)
 
 
 def test_ensemble_trajectories_match_oracle_snapshots():

@@ -145,7 +145,7 @@ def test_dimer_closed_forms():
 
 
 def test_mixture_pair_parameters():
-    """for-a-lj-n-pbc-m.f:323-338: Lorentz-Berthelot, eps12 = sqrt(eps1 eps2)."""
+    """for-a-lj-n-pbc-m.f:61-76: Lorentz-Berthelot, eps12 = sqrt(eps1 eps2)."""
     s, e, m = cases.SIG_AR, cases.EPS_AR, cases.MAS_AR
     c = _dimer(1.2)
     c.update(nsorts=2, api=np.array([1, 2], dtype=np.int32), num=np.array([1, 1]),
@@ -159,6 +159,91 @@ def test_mixture_pair_parameters():
     assert o.arr["ax"][1] == pytest.approx(-f / (2 * m), rel=1e-11)
     r = 1.2 * s / s12
     assert o.st.epot == pytest.approx(4 * e12 * (r ** -12 - r ** -6), rel=1e-11)
+
+
+def test_virial_and_pressure_closed_forms():
+    """Pair virial W = r F(r) of the dimer (for-a-lj-n-pbc.f:106,114) in the
+    one-species, Lorentz-Berthelot and pair-table forms; pressure
+    1e13/V (N k T - W/3) (:116-117)."""
+    s, e, m = cases.SIG_AR, cases.EPS_AR, cases.MAS_AR
+    for r in (0.93, 2 ** (1 / 6), 1.3, 2.2, 2.49):
+        c = _dimer(r)
+        o = po.Oracle(c, "mav")
+        o.neighbor(0)
+        o.accatom()
+        w = lj_force(r * s, s, e) * r * s
+        assert o.st.virial == pytest.approx(w, rel=1e-11, abs=1e-12 * 24 * e)
+        V = float(np.prod(c["box"]))
+        assert o.st.pressure == pytest.approx(1e13 / V * (2 * cases.CIKB * c["att"] - w / 3), rel=1e-11)
+    # beyond the cutoff: no virial
+    o = po.Oracle(_dimer(2.6), "mav")
+    o.neighbor(0)
+    o.accatom()
+    assert o.st.virial == 0.0
+    # A-B pair at eps_AB = 0.3, sigma_AB = 0.9 sigma through the pair table
+    c = _dimer(1.2)
+    c.update(nsorts=2, api=np.array([1, 2], dtype=np.int32), num=np.array([1, 1]),
+             mas=np.array([m, m]), sig=np.array([s, s]), eps=np.array([e, 0.1 * e]))
+    tab_e = [[e, 0.3 * e], [0.3 * e, 0.1 * e]]
+    tab_s = [[s, 0.9 * s], [0.9 * s, s]]
+    o = po.Oracle(c, "nuc", eps_ab=tab_e, sig_ab=tab_s)
+    o.neighbor(0)
+    o.accatom()
+    f = lj_force(1.2 * s, 0.9 * s, 0.3 * e)
+    assert o.arr["ax"][0] == pytest.approx(f / m, rel=1e-11)
+    assert o.st.virial == pytest.approx(f * 1.2 * s, rel=1e-11)
+    q = 1.2 / 0.9
+    assert o.st.epot == pytest.approx(4 * 0.3 * e * (q ** -12 - q ** -6), rel=1e-11)
+
+
+def test_virial_theorem_for_a_free_cluster():
+    """For a cluster whose pairs never cross the periodic faces,
+    sum over pairs r_ij . f_ij = sum over atoms r_i . F_i (F_i = m a_i): the
+    virial from the pair terms equals the one from the forces."""
+    c = cases.make_case(n=300, rho=0.05, seed=31, dense_frac=1.0, ncenters=1, dense_rho=0.7)
+    L = c["box"][0]
+    for k in ("x", "y", "z"):  # move the droplet to the box centre
+        v = c[k] - c[k][0]
+        v -= L * np.round(v / L)
+        c[k] = v + 0.5 * L
+    for variant, kw in (("mav", {}), ("nuc", dict(nsorts=2))):
+        cc = dict(c)
+        if kw:
+            cc = cases.make_case(n=300, rho=0.05, seed=31, dense_frac=1.0, ncenters=1, dense_rho=0.7,
+                                 nsorts=2, frac_a=0.5)
+            for k in ("x", "y", "z"):
+                cc[k] = c[k]
+        o = po.Oracle(cc, variant)
+        o.neighbor(0)
+        o.accatom()
+        pos = np.stack([o.arr["x"], o.arr["y"], o.arr["z"]], 1)
+        span = pos.max(0) - pos.min(0)
+        assert np.all(span < L - 2 * cc["atrskin"])  # no pair across a face
+        mass = np.asarray(cc["mas"])[o.arr["api"] - 1]
+        acc = np.stack([o.arr["ax"], o.arr["ay"], o.arr["az"]], 1)
+        w_forces = float((pos * acc * mass[:, None]).sum())
+        assert o.st.virial == pytest.approx(w_forces, rel=1e-9)
+        assert abs(o.st.virial) > 0
+
+
+def test_pair_table_with_lorentz_berthelot_values_is_the_m_form():
+    """ORC_ACC_TABLE holding the LB values reproduces for-a-lj-n-pbc-m.f
+    (forces, energy and virial) bit for bit."""
+    c = cases.make_case(n=500, rho=0.05, seed=33, dense_frac=0.4, nsorts=2, frac_a=0.3)
+    s1, s2 = c["sig"]
+    e1, e2 = c["eps"]
+    tab_e = [[e1, np.sqrt(e1 * e2)], [np.sqrt(e1 * e2), e2]]
+    tab_s = [[s1, 0.5 * (s1 + s2)], [0.5 * (s1 + s2), s2]]
+    out = []
+    for kw in ({}, dict(eps_ab=tab_e, sig_ab=tab_s)):
+        o = po.Oracle(c, "nuc", **kw)
+        o.neighbor(0)
+        o.accatom()
+        out.append(o)
+    for k in ("ax", "ay", "az"):
+        np.testing.assert_array_equal(out[0].arr[k], out[1].arr[k])
+    assert out[0].st.epot == out[1].st.epot
+    assert out[0].st.virial == out[1].st.virial
 
 
 def _brute_pairs(case, radius):
