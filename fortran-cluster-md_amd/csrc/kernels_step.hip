@@ -524,7 +524,8 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
 }
 
 // Mixed: separations from the FP32 cell-offset records (make_posf: one 16-byte
-// gather per neighbour, no FP64): the cell difference wrapped to a neighbour
+// gather per neighbour, no FP64 except the cutoff test of the rare pair within
+// 1e-5 of the cutoff, which is the reference's): the cell difference wrapped to a neighbour
 // (sep_wrap; every listed pair was in neighbouring cells at the list build),
 // or for a box of fewer than 3 cells on some side (SMALL, a loop of its own)
 // the rounding image.  Pair distance and LJ terms in FP32 with fused
@@ -533,6 +534,23 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
 // what an FP32 one does here, and fewer instructions than a compensated FP32
 // sum).  The single-species form works in sigma units, so that its constants
 // are instruction immediates.
+// The reference's cutoff test of one pair on the FP64 positions
+// (for-a-lj-n-pbc.f:89-95): its minimum image d + L (bf1 - int(bf2 + d/L)),
+// r2 <= atrcut^2.  For the rare pair near the cutoff on the mixed path.
+__device__ __forceinline__ bool reference_cut_test(const DevPtrs &d, const TrajParams &tp,
+                                                   const TrajScalars &ts, int i, int j, double rc2)
+{
+    const double bf1 = 2.0 + fint(ts.atrskin / tp.atr);
+    const double bf2 = bf1 + 0.5;
+    double dx = d.x[i] - d.x[j];
+    double dy = d.y[i] - d.y[j];
+    double dz = d.z[i] - d.z[j];
+    dx = dx + tp.box[0] * (bf1 - fint(bf2 + dx * tp.ibox[0]));
+    dy = dy + tp.box[1] * (bf1 - fint(bf2 + dy * tp.ibox[1]));
+    dz = dz + tp.box[2] * (bf1 - fint(bf2 + dz * tp.ibox[2]));
+    return dx * dx + dy * dy + dz * dz <= rc2;
+}
+
 template <int FORM, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
@@ -547,12 +565,70 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
     const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
     const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
-    const float rc2f = (float)(tp.atrcut * tp.atrcut);
+    // cutoff decision band: r2 from the cell-offset records is within
+    // ~2e-6 relative of the reference's FP64 r2, so outside +-1e-5 of rc^2 the
+    // FP32 verdict is the reference's; inside, the reference's FP64 test
+    // decides (the truncated force jumps at rc, so one pair decided the other
+    // way would be the largest error of a large system)
+    const double rc2 = tp.atrcut * tp.atrcut;
+    const float in2 = (float)(rc2 * (1.0 - 1e-5)), out2 = (float)(rc2 * (1.0 + 1e-5));
     const float sig2 = tp.sig2f;  // sigma^2 of the single-species form
     const int nn = h.nn;
     double sfx = 0.0, sfy = 0.0, sfz = 0.0;
+    // this atom's energy and virial terms, summed in FP32 (its own few
+    // pairs) and reduced over atoms in FP64 by the caller
+    float ef = 0.0f, wf = 0.0f;
+    // the FP32 separation of i from the neighbour record q (its cell
+    // difference wrapped to a neighbour) and its square
+    auto separation = [&](const float4 &q, float &dxf, float &dyf, float &dzf) {
+        const float cxj = cell_xf(q.w), cyj = cell_yf(q.w), czj = cell_zf(q.w);
+        if (SMALL) {
+            dxf = ncx < 3.0f ? sep_small(cxi, cxj, sx, pi.x - q.x, bx, ibx)
+                             : sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - q.x);
+            dyf = ncy < 3.0f ? sep_small(cyi, cyj, sy, pi.y - q.y, by, iby)
+                             : sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - q.y);
+            dzf = ncz < 3.0f ? sep_small(czi, czj, sz, pi.z - q.z, bz, ibz)
+                             : sep_wrap(czi, czj, ncz, icz, sz, pi.z - q.z);
+        } else {
+            dxf = sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - q.x);
+            dyf = sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - q.y);
+            dzf = sep_wrap(czi, czj, ncz, icz, sz, pi.z - q.z);
+        }
+        return __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
+    };
+    // one pair's force, energy and virial terms added to the sums (sgn = 1),
+    // or taken out again (sgn = -1)
+    auto pair_terms = [&](float dxf, float dyf, float dzf, float r2, int tjj, float sgn) {
+        const float ri2 = __builtin_amdgcn_rcpf(r2);
+        float fij;
+        if (FORM == ACC_1) {
+            // in sigma units, s6 = (sigma/r)^6: r^-2 r^-6 (sigma^12 r^-6 -
+            // sigma^6/2) = r^-2 s6 (s6 - 1/2), the energy term r^-6
+            // (sigma^12 r^-6 - sigma^6) = s6 (s6 - 1) and the virial term
+            // r^-6 (sigma^12 r^-6 - sigma^6/2) = s6 (s6 - 1/2); the constants
+            // are immediates
+            const float u2 = sig2 * ri2;
+            const float s6 = u2 * u2 * u2;
+            const float t = s6 - 0.5f;
+            fij = ri2 * s6 * t;
+            ef = __builtin_fmaf(sgn * s6, s6 - 1.0f, ef);
+            wf = __builtin_fmaf(sgn * s6, t, wf);
+        } else {
+            const float r6 = ri2 * ri2 * ri2;
+            const int pij = ti * MAXS + tjj;
+            const float q = __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
+            fij = (float)tp.e48[pij] * ri2 * r6 * q;
+            ef = __builtin_fmaf(sgn * (float)tp.e4[pij], r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]), ef);
+            wf = __builtin_fmaf(sgn * fij, r2, wf);
+        }
+        const double fd = (double)(sgn * fij);
+        sfx = __builtin_fma((double)dxf, fd, sfx);
+        sfy = __builtin_fma((double)dyf, fd, sfy);
+        sfz = __builtin_fma((double)dzf, fd, sfz);
+    };
     int jj[U], tj[U];
     float4 pj[U];
+    float r2max = 0.0f;  // the largest r2 taken: above in2, a pair in the band was
     auto load = [&](int s0) {
         h.batch(stage, s0, jj);
 #pragma unroll
@@ -565,56 +641,31 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (s0 + u >= nn) break;
-            const float cxj = cell_xf(pj[u].w), cyj = cell_yf(pj[u].w), czj = cell_zf(pj[u].w);
             float dxf, dyf, dzf;
-            if (SMALL) {
-                dxf = ncx < 3.0f ? sep_small(cxi, cxj, sx, pi.x - pj[u].x, bx, ibx)
-                                 : sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - pj[u].x);
-                dyf = ncy < 3.0f ? sep_small(cyi, cyj, sy, pi.y - pj[u].y, by, iby)
-                                 : sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - pj[u].y);
-                dzf = ncz < 3.0f ? sep_small(czi, czj, sz, pi.z - pj[u].z, bz, ibz)
-                                 : sep_wrap(czi, czj, ncz, icz, sz, pi.z - pj[u].z);
-            } else {
-                dxf = sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - pj[u].x);
-                dyf = sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - pj[u].y);
-                dzf = sep_wrap(czi, czj, ncz, icz, sz, pi.z - pj[u].z);
-            }
-            const float r2 = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
-            if (r2 <= rc2f) {
-                const float ri2 = __builtin_amdgcn_rcpf(r2);
-                float fij;
-                if (FORM == ACC_1) {
-                    // in sigma units, s6 = (sigma/r)^6: the virial term
-                    // r^-6 (sigma^12 r^-6 - sigma^6/2) = s6^2 - s6/2, the force
-                    // r^-2 s6 (s6 - 1/2) = r^-2 times that, the energy term
-                    // r^-6 (sigma^12 r^-6 - sigma^6) = s6^2 - s6; the
-                    // constants are immediates
-                    const float u2 = sig2 * ri2;
-                    const float s6 = u2 * u2 * u2;
-                    const float s12 = s6 * s6;
-                    const float wt = __builtin_fmaf(s6, -0.5f, s12);
-                    fij = ri2 * wt;
-                    e += (double)(s12 - s6);
-                    w += (double)wt;
-                } else {
-                    const float r6 = ri2 * ri2 * ri2;
-                    const int pij = ti * MAXS + tj[u];
-                    const float wt = r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
-                    fij = (float)tp.e48[pij] * ri2 * wt;
-                    e += tp.e4[pij] * (double)(r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]));
-                    w = __builtin_fma(tp.e48[pij], (double)wt, w);
-                }
-                const double fd = (double)fij;
-                sfx = __builtin_fma((double)dxf, fd, sfx);
-                sfy = __builtin_fma((double)dyf, fd, sfy);
-                sfz = __builtin_fma((double)dzf, fd, sfz);
+            const float r2 = separation(pj[u], dxf, dyf, dzf);
+            if (r2 <= out2) {
+                pair_terms(dxf, dyf, dzf, r2, tj[u], 1.0f);
+                r2max = fmaxf(r2max, r2);
             }
         }
     };
     walk_list<U>(stage, h, load, sum);
+    // the rare atom that took a pair in the band walks its list once more
+    // and takes out again each such pair the reference's FP64 test rejects
+    if (r2max > in2) {
+        for (int q = 0; q < nn; ++q) {
+            const int j = h.decode(q < STAGE ? stage[q][threadIdx.x] : h.word(q));
+            float dxf, dyf, dzf;
+            const float r2 = separation(d.posf[j], dxf, dyf, dzf);
+            if (r2 <= out2 && r2 > in2 && !reference_cut_test(d, tp, ts, i, j, rc2))
+                pair_terms(dxf, dyf, dzf, r2, (FORM == ACC_1) ? 0 : d.typ[j], -1.0f);
+        }
+    }
     fx = sfx;
     fy = sfy;
     fz = sfz;
+    e += (double)ef;
+    w += (double)wf;
 }
 
 template <typename AccT, int FORM, int U, typename LT>
@@ -678,17 +729,20 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
         const int ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
         // the velocities, loaded before the pair sum so that they arrive
         // while it runs
+        double fx, fy, fz;
+        if (form1)
+            pair_sum<AccT, ACC_1, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+        else
+            pair_sum<AccT, ACC_M, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+        // the velocities after the pair sum: loaded before it, they would
+        // hold six registers through the pair loop, which the single-species
+        // kernel (held to 72) would spill
         double vx = 0.0, vy = 0.0, vz = 0.0;
         if (mode == FORCE_STEP) {
             vx = d.vx[i];
             vy = d.vy[i];
             vz = d.vz[i];
         }
-        double fx, fy, fz;
-        if (form1)
-            pair_sum<AccT, ACC_1, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
-        else
-            pair_sum<AccT, ACC_M, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
         // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:146-152
         double ax, ay, az;
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;

@@ -271,6 +271,14 @@ class System:
             out[k] = int(v)
         return out
 
+    def write_restart(self, path, traj: int = 0, mxa: int | None = None):
+        """Trajectory traj as a restart file the reference's readRST reads
+        (wrt-rst.f:30-45 layout, mxa-long records; mxa defaults to natom)."""
+        from . import restart
+        st = self.download(traj)
+        sc = self.scalars(traj)
+        restart.write_restart(path, st, mxa or self.natom[traj], ekin=sc.ekin, epot=sc.epot)
+
     def halflist(self, traj: int = 0):
         n = self.natom[traj]
         sc = self.scalars(traj)

@@ -123,6 +123,18 @@ module fcmd_capi
        real(c_double) :: cek(*), ceks(*), cekf(*), ceksf(*)
      end function fcmd_stoddard
 
+     integer(c_int) function fcmd_init_random(seed, natom, box, rmin, att, mas, api, &
+          x, y, z, vx, vy, vz) bind(C, name='fcmd_init_random')
+       import :: c_int, c_double, c_long_long
+       integer(c_long_long), value :: seed
+       integer(c_int), value :: natom
+       real(c_double) :: box(3)
+       real(c_double), value :: rmin, att
+       real(c_double) :: mas(*)
+       integer(c_int) :: api(*)
+       real(c_double) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+     end function fcmd_init_random
+
      integer(c_int) function fcmd_halflist(sys, traj, atnlist, cap, atnidx, length) &
           bind(C, name='fcmd_halflist')
        import :: c_int, c_ptr, c_long_long

@@ -9,9 +9,10 @@ read-rst.f:37-49 (unformatted sequential; the record arrays are mxa long).
 from __future__ import annotations
 
 import os
-import struct
 
 import numpy as np
+
+from fcmd import restart
 
 CIKB = 1.38066e-6
 CIMU = 1.66056
@@ -35,9 +36,13 @@ def _i(v):
 
 
 def write_deck(path, case, ntim, ftcsi, ftave, sctim, ethst="-", q_star=100.0, fpath="./",
-               rst="start.rst", nbtim=10):
+               rst="start.rst", nbtim=10, ftxyz=0, ftvel=0, ftacc=0, ftrst=0, restart=True,
+               rndctrl="-", nucleation=None):
     """case in internal units (tests/cases.py layout); written back in deck
-    units (nm, u, K, fs) so that read-3d.f reproduces the internal values."""
+    units (nm, u, K, fs) so that read-3d.f reproduces the internal values.
+    restart: start from the restart file rst (read-rst.f), else from the
+    linked initconf; nucleation: (csover, clwait) switches the
+    chk-nucleation.f abort on."""
     sig1 = case["sig"][0]
     rule = "---------1---------2---------3---------4---------5---------6---------7---------8\n"
     L = []
@@ -45,7 +50,7 @@ def write_deck(path, case, ntim, ftcsi, ftave, sctim, ethst="-", q_star=100.0, f
     L.append(_val("Version number Program/Inputfile", " 1.30"))
     L.append(rule)
     L.append(_val("path for following files (relative)", fpath))
-    for name, ft in (("test.xyz", 0), ("test.vel", 0), ("test.acc", 0), ("test.rst", 0),
+    for name, ft in (("test.xyz", ftxyz), ("test.vel", ftvel), ("test.acc", ftacc), ("test.rst", ftrst),
                      ("test.csi", ftcsi), ("test.ave", ftave), ("test.rat", 0)):
         L.append(_val("filename", name))
         L.append(_val("output interval", _i(ft)))
@@ -69,11 +74,11 @@ def write_deck(path, case, ntim, ftcsi, ftave, sctim, ethst="-", q_star=100.0, f
         L.append(_val("maximum resize-step", _f(0.04)))
     L.append(_val("time steps between resizes", _i(50)))
     L.append(rule)
-    L.append(_val("use restartfile? (X = yes, - = no)", "X"))
+    L.append(_val("use restartfile? (X = yes, - = no)", "X" if restart else "-"))
     L.append(_val("file name of Input-Restartfile", rst))
     L.append(_val("thermostat? (X,N = yes, - = no)", ethst))
     L.append(_val("Nose-Hoover coupling constant Q", _f(q_star)))
-    L.append(_val("pure random velocities after init", "-"))
+    L.append(_val("pure random velocities after init", rndctrl))
     L.append(_val("Cluster Thermostat", "-"))
     L.append(_val("Cluster Thermostat Timer", _i(200)))
     L.append(rule)
@@ -85,9 +90,10 @@ def write_deck(path, case, ntim, ftcsi, ftave, sctim, ethst="-", q_star=100.0, f
     L.append(_val("update-Time for Neighbor", _i(nbtim)))
     L.append(_val("update-Time for Natom-change", _i(500)))
     L.append(rule)
-    L.append(_val("check for nucleation?", "-"))
-    L.append(_val("minimum cluster size to abort", _i(15)))
-    L.append(_val("time steps the cluster must exist", _i(10000)))
+    csover, clwait = nucleation if nucleation else (15, 10000)
+    L.append(_val("check for nucleation?", "X" if nucleation else "-"))
+    L.append(_val("minimum cluster size to abort", _i(csover)))
+    L.append(_val("time steps the cluster must exist", _i(clwait)))
     L.append(rule)
     L.append(_val("number of atomsorts", _i(int(case["nsorts"]))))
     L.append(rule)
@@ -126,29 +132,8 @@ def deck_units(case):
 
 
 def write_restart(path, case, mxa=6000, ekin=0.0, epot=0.0):
-    """Unformatted sequential records as wrt-rst.f:34-43 writes them."""
-    n = int(case["natom"])
-
-    def rec(f, payload):
-        f.write(struct.pack("<i", len(payload)))
-        f.write(payload)
-        f.write(struct.pack("<i", len(payload)))
-
-    def arr(v):
-        a = np.zeros(mxa)
-        a[:n] = v
-        return a.astype("<f8").tobytes()
-
-    with open(path, "wb") as f:
-        rec(f, struct.pack("<i", mxa))
-        rec(f, struct.pack("<5d", ekin, epot, ekin, epot, ekin))
-        rec(f, struct.pack("<4d", 0.0, 0.0, 0.0, 0.0))
-        rec(f, arr(case["x"]) + arr(case["y"]) + arr(case["z"]))
-        rec(f, arr(case["vx"]) + arr(case["vy"]) + arr(case["vz"]))
-        rec(f, arr(case["ax"]) + arr(case["ay"]) + arr(case["az"]))
-        z = arr(np.zeros(n))
-        for _ in range(3):
-            rec(f, z + z + z)
+    """The package's restart writer (wrt-rst.f:30-45 layout) on a case."""
+    restart.write_restart(path, {k: case[k] for k in restart.KEYS}, mxa, ekin=ekin, epot=epot)
 
 
 def read_csi(path):

@@ -13,7 +13,9 @@ import pytest
 
 import cases
 import decks
+import fcmd
 import pyoracle as po
+from fcmd import restart
 
 pytestmark = pytest.mark.gpu
 
@@ -24,23 +26,28 @@ def _have(name):
     return os.path.exists(os.path.join(BIN, name))
 
 
-def _run_pair(tmp_path, variant, case, ntim, ftcsi, ftave, ethst="-", env=None):
+def _run_pair(tmp_path, variant, case, ntim, ftcsi, ftave, ethst="-", env=None, progs=None, **deck):
     o = po.Oracle(case, "mav" if variant == "mav" else "nuc")
     o.neighbor(0)
     o.accatom()
     c2 = dict(case, ax=o.arr["ax"], ay=o.arr["ay"], az=o.arr["az"])
     out = {}
-    for prog in (f"cluster_{variant}", f"cluster_{variant}_hip"):
+    progs = progs or (f"cluster_{variant}", f"cluster_{variant}_hip")
+    for prog in progs:
         d = tmp_path / prog
         d.mkdir()
+        kw = dict(deck)
+        if prog.endswith("_rnd_hip"):
+            kw["restart"] = False  # started by the seeded-vapour initconf
         decks.write_deck(str(d / "t.3d"), c2, ntim=ntim, ftcsi=ftcsi, ftave=ftave, sctim=ftcsi,
-                         ethst=ethst)
-        decks.write_restart(str(d / "start.rst"), c2)
+                         ethst=ethst, **kw)
+        # read-rst.f takes epot from the file (no AccAtom before step 0)
+        decks.write_restart(str(d / "start.rst"), c2, epot=o.st.epot)
         r = decks.run_program(os.path.join(BIN, prog), "t.3d", str(d), env=env)
         assert r.returncode == 0, (prog, r.stdout[-2000:], r.stderr[-2000:])
-        out[prog] = dict(stdout=r.stdout, csi=decks.read_csi(str(d / "test.csi")),
+        out[prog] = dict(stdout=r.stdout, csi=decks.read_csi(str(d / "test.csi")) if ftcsi else {},
                          ave=decks.read_ave(str(d / "test.ave")), dir=d)
-    return out[f"cluster_{variant}"], out[f"cluster_{variant}_hip"]
+    return out[progs[0]], out[progs[1]]
 
 
 def _rebuild_lines(stdout):
@@ -108,3 +115,94 @@ def test_dropin_fre(tmp_path):
     want = {int(k): int(o.arr["csi"][k]) for k in np.nonzero(o.arr["csi"][1:])[0] + 1}
     got = {k: v[0] for k, v in hip["csi"][0].items()}
     assert got == want
+
+
+def _frames(path, first_step=0):
+    """{step: (n, 3) array} from a wrt-xyz / wrt-vel / wrt-acc file.  Frames
+    open with two numeric lines (natom, step), except wrt-xyz.f's at the
+    start step with box corners off, which has none (wrt-xyz.f:41-55)."""
+    out, hdr, rows, step = {}, [], [], first_step
+    for ln in open(path):
+        p = ln.split()
+        if not p:
+            continue
+        if len(p) == 1:
+            if rows:
+                out[step] = np.array(rows)
+                rows = []
+            hdr.append(int(p[0]))
+            if len(hdr) == 2:
+                step = hdr[1]
+                hdr = []
+        else:
+            rows.append([float(v) for v in p[-3:]])
+    if rows:
+        out[step] = np.array(rows)
+    return out
+
+
+@pytest.mark.skipif(not (_have("cluster_mav") and _have("cluster_mav_hip")), reason="programs not built")
+def test_dropin_writers_and_nucleation_abort(tmp_path):
+    """With the drop-in resident on the device, main.f's own writers
+    (wrtXYZ/VEL/ACC/RST, main.f:121-132) and chk_nucleation (:157-162)
+    still read the current state: positions, velocities and accelerations
+    written every 40 steps and the restart file agree with the reference
+    program's, and a cluster of 2 or more atoms that persists 100 steps
+    stops both programs at the same step (chk-nucleation.f:42-56)."""
+    case = decks.deck_units(cases.make_case(n=1000, rho=0.02, seed=24, dense_frac=0.3, ncenters=4))
+    ref, hip = _run_pair(tmp_path, "mav", case, ntim=400, ftcsi=100, ftave=50,
+                         ftxyz=40, ftvel=40, ftacc=40, ftrst=100, nucleation=(2, 100))
+    _compare(ref, hip)
+    for out in (ref, hip):
+        assert "ABORTING simulation at timestep:" in out["stdout"]
+    stop = [ln for ln in ref["stdout"].splitlines() if "ABORTING" in ln]
+    assert stop == [ln for ln in hip["stdout"].splitlines() if "ABORTING" in ln]
+    assert int(stop[0].split(":")[1].split()[0]) == 200
+    assert ref["ave"][-1][0] == 200
+    L = case["box"][0]
+    for name, atol in (("test.xyz", 2.0e-4), ("test.vel", 2e-10), ("test.acc", 2e-10)):
+        a, b = _frames(ref["dir"] / name), _frames(hip["dir"] / name)
+        assert sorted(a) == sorted(b) == list(range(0, 201, 40)), name
+        for st in a:
+            d = np.abs(a[st] - b[st])
+            if name == "test.xyz":  # wrapped at different rebuilds: compare modulo the box
+                d = np.minimum(d, np.abs(d - L))
+            assert d.max() <= atol + 1e-8 * np.abs(a[st]).max(), (name, st, d.max())
+    ra = restart.read_restart(str(ref["dir"] / "test.rst"), case["natom"])
+    rb = restart.read_restart(str(hip["dir"] / "test.rst"), case["natom"])
+    assert ra["mxa"] == rb["mxa"] == 6000
+    for k in restart.KEYS:
+        scale = np.abs(ra[k]).max()
+        d = np.abs(ra[k] - rb[k])
+        if k in ("x", "y", "z"):
+            d = np.minimum(d, np.abs(d - L))
+        assert d.max() <= 1e-9 * scale, k
+
+
+@pytest.mark.skipif(not (_have("cluster_nose") and _have("cluster_nose_rnd_hip")), reason="programs not built")
+def test_dropin_initconf_seeded_vapour(tmp_path):
+    """The seeded-vapour initconf drop-in (alongside init-cube1.f:33-128):
+    the state it hands main.f is fcmd_init_random's, bit for bit (the
+    restart written at step 0), and the run from it matches the reference
+    program started from the same state through a restart file written by
+    the package's writer (read by read-rst.f)."""
+    case = decks.deck_units(cases.make_case(n=800, rho=0.03, seed=25, ethst_nh=1))
+    seed = 12345
+    api = np.asarray(case["api"], dtype=np.int32)
+    x, y, z, vx, vy, vz = fcmd.init_random(seed, case["natom"], case["box"], 0.9 * case["sig"][0],
+                                           case["att"], case["mas"], api)
+    c0 = dict(case, x=x, y=y, z=z, vx=vx, vy=vy, vz=vz)
+    ref, hip = _run_pair(tmp_path, "nose", c0, ntim=300, ftcsi=100, ftave=50, ethst="N",
+                         env={"FCMD_SEED": str(seed)}, progs=("cluster_nose", "cluster_nose_rnd_hip"),
+                         ftrst=1000)
+    line = [ln for ln in hip["stdout"].splitlines() if "seeded random vapour, seed" in ln]
+    assert line and int(line[0].split()[-1]) == seed
+    st = restart.read_restart(str(hip["dir"] / "test.rst"), case["natom"])
+    # positions as neighbor(0) wrapped them (dmod(dmod(x,L)+L,L): the
+    # generator's x to within an ulp of L), velocities bit for bit
+    L = case["box"][0]
+    for k, v in (("x", x), ("y", y), ("z", z)):
+        assert np.abs(st[k] - v).max() <= 4e-16 * L, k
+    for k, v in (("vx", vx), ("vy", vy), ("vz", vz)):
+        np.testing.assert_array_equal(st[k], v, err_msg=k)
+    _compare(ref, hip)
