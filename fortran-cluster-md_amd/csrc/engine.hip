@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -109,6 +110,45 @@ __global__ __launch_bounds__(256) void k_first_err(const TrajScalars *ts, int nt
     if (threadIdx.x == 0) *out = first == INT_MAX ? 0 : ts[first].err;
 }
 
+// k_fill: n ints of p set to v
+__global__ void k_fill(int *p, int n, int v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// Scratch of the allocation-free run (fcmd_run_noalloc): its first-passage
+// times and largest-cluster series are staged in this fixed device array
+// instead of buffers the library allocates.  Every GPU holds its own copy at
+// its own address, kept until the process ends; host code never passes the
+// array's name to a kernel or a copy (in host code it is a stand-in, not a
+// device address) but the address hipGetSymbolAddress gave for the device
+// (scratch_ptr, set up once per device by fcmd_create).
+static __device__ __align__(256) int scratch[FCMD_NOALLOC_INTS];
+
+constexpr int MAX_DEVICES = 64;
+int *scratch_ptr[MAX_DEVICES] = {};
+std::mutex scratch_mu;
+
+// the scratch address on the current device (looked up once per device)
+int scratch_on_current_device(int **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return FCMD_ERR_HIP;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "fcmd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return FCMD_ERR_HIP;
+        }
+        scratch_ptr[dev] = (int *)p;
+    }
+    *out = scratch_ptr[dev];
+    return FCMD_OK;
+}
+
 }  // namespace
 
 namespace fcmd {
@@ -149,6 +189,7 @@ struct fcmd_system {
     // device, kept between runs and grown when a run needs more
     int *dseries = nullptr, *dfpt = nullptr;
     long long series_cap = 0;
+    int *scratch = nullptr;     // this device's fixed scratch (fcmd_run_noalloc)
     // HIP events around the first NAN_TIMED analyses of the last run
     static constexpr int NAN_TIMED = 16;
     hipEvent_t an_ev[2 * NAN_TIMED] = {};
@@ -370,6 +411,9 @@ const char *fcmd_error_text(int code)
     case FCMD_ERR_NH: return "Numerical integration did not converge!";
     case FCMD_ERR_EDGES: return "*** FATAL ERROR *** too many cluster neighbours per atom";
     case FCMD_ERR_CELLS: return "*** FATAL ERROR *** cell grid exceeds its allocation";
+    case FCMD_ERR_SCRATCH:
+        return "run too large for the allocation-free entry point: (analyses + 1) x trajectories "
+               "exceeds FCMD_NOALLOC_INTS";
     case FCMD_ERR_ARG: return "invalid argument";
     case FCMD_ERR_HIP: return "HIP runtime error";
     case FCMD_ERR_NODEVICE: return "no HIP device";
@@ -384,6 +428,11 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     if (fcmd_device_count() <= 0) return FCMD_ERR_NODEVICE;
     fcmd_system *s = new (std::nothrow) fcmd_system();
     if (!s) return FCMD_ERR_ARG;
+    // set-up of the allocation-free run's scratch on this device
+    if (int rc0 = scratch_on_current_device(&s->scratch)) {
+        delete s;
+        return rc0;
+    }
     s->ntraj = ntraj;
     s->mixed = precision == FCMD_MIXED;
     s->list_width = list_width > 0 ? list_width : 128;
@@ -891,6 +940,49 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
         if (fpt) HIPCHK(hipMemcpy(fpt, s->dfpt, sizeof(int) * s->ntraj, hipMemcpyDeviceToHost));
     }
     return sync_err(s);
+}
+
+int fcmd_run_noalloc(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series, int *fpt)
+{
+    if (!s || nsteps < 0 || cluster_every < 0) return FCMD_ERR_ARG;
+    if (!s->scratch) return FCMD_ERR_HIP;
+    const int nsnap = cluster_every > 0 ? (nsteps + cluster_every - 1) / cluster_every : 0;
+    if ((long long)(nsnap + 1) * s->ntraj > FCMD_NOALLOC_INTS) return FCMD_ERR_SCRATCH;
+    // scratch layout: fpt[ntraj], then series[nsnap][ntraj]; both written
+    // before they are read (fpt from the caller or -1, series by each
+    // analysis for every trajectory), whatever an earlier call left there
+    int *dfpt = s->scratch, *dser = s->scratch + s->ntraj;
+    if (nsnap > 0) {
+        if (fpt)
+            HIPCHK(hipMemcpyAsync(dfpt, fpt, sizeof(int) * s->ntraj, hipMemcpyHostToDevice, s->stream));
+        else
+            hipLaunchKernelGGL(k_fill, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, dfpt, s->ntraj, -1);
+        set_want(s, -1);
+    }
+    int done = 0, snap = 0;
+    while (done < nsteps) {
+        if (nsnap > 0 && done % cluster_every == 0) {
+            finalize(s);
+            launch_cluster(s->d, s->g, s->any_fre, false, s->stream);
+            hipLaunchKernelGGL(k_record, dim3((s->ntraj + 255) / 256), dim3(256), 0, s->stream, s->d.cls, s->d.ts,
+                               s->ntraj, dser + (size_t)snap * s->ntraj, dfpt, fpt_size);
+            if (s->count_nmax > 0) launch_size_counts(s->d, s->g, s->count_nmax, s->count_warmup, s->stream);
+            ++snap;
+        }
+        const int todo = std::min(nsteps - done, nsnap > 0 ? cluster_every - done % cluster_every : nsteps - done);
+        for (int k = 0; k < todo; ++k)
+            if (int rc = enqueue_step(s)) return rc;
+        HIPCHK(hipGetLastError());
+        done += todo;
+    }
+    finalize(s);
+    if (nsnap > 0) {
+        if (series)
+            HIPCHK(hipMemcpyAsync(series, dser, sizeof(int) * (size_t)nsnap * s->ntraj, hipMemcpyDeviceToHost,
+                                  s->stream));
+        if (fpt) HIPCHK(hipMemcpyAsync(fpt, dfpt, sizeof(int) * s->ntraj, hipMemcpyDeviceToHost, s->stream));
+    }
+    return sync_err(s);  // also waits for the copies above
 }
 
 int fcmd_size_counts(fcmd_system *s, int nmax, int warmup)

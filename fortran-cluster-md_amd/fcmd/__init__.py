@@ -89,7 +89,7 @@ EXPORTS = ("fcmd_device_count", "fcmd_error_text", "fcmd_create", "fcmd_destroy"
            "fcmd_stoddard", "fcmd_halflist", "fcmd_run", "fcmd_init_random",
            "fcmd_last_run_ms", "fcmd_time_kernels", "fcmd_list_entries", "fcmd_set_tuning",
            "fcmd_get_tuning", "fcmd_autotune", "fcmd_size_counts", "fcmd_get_size_counts",
-           "fcmd_last_run_analysis_ms")
+           "fcmd_last_run_analysis_ms", "fcmd_run_noalloc")
 
 
 def lib():

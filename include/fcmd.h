@@ -45,6 +45,7 @@ enum {
     FCMD_ERR_NH = 3,        /* int-verlet-nh-mav.f:151 "did not converge" */
     FCMD_ERR_EDGES = 4,     /* more cluster edges per atom than the engine keeps */
     FCMD_ERR_CELLS = 5,     /* cell grid larger than allocated */
+    FCMD_ERR_SCRATCH = 6,   /* fcmd_run_noalloc: the run needs more than FCMD_NOALLOC_INTS */
     FCMD_ERR_ARG = 10,
     FCMD_ERR_HIP = 11,
     FCMD_ERR_NODEVICE = 12
@@ -162,6 +163,23 @@ int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *at
  * largest clusters only; fcmd_stoddard makes the per-atom arrays on demand. */
 int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series,
              int *fpt, int use_graph);
+
+/* fcmd_run without any allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Same steps, analyses,
+ * outputs and bits as fcmd_run(..., use_graph = 0): every step is launched
+ * on the stream (no graph is captured or instantiated), and the run's
+ * first-passage times and largest-cluster series are staged in a fixed
+ * device array of FCMD_NOALLOC_INTS ints that each GPU holds once for the
+ * process (fcmd_create finds its address), instead of buffers fcmd_run
+ * allocates.  A run needs (ceil(nsteps / cluster_every) + 1) x ntraj ints
+ * of it; a larger one returns FCMD_ERR_SCRATCH before doing anything.  The
+ * array keeps the last call's contents; every call writes what it reads.
+ * Calls on systems of the same GPU share the array: calls that overlap in
+ * time on one GPU race and must not be made. */
+#define FCMD_NOALLOC_INTS 65536
+int fcmd_run_noalloc(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *series,
+                     int *fpt);
 
 /* Seeded initial state: natom positions uniform in the box with no pair
  * closer than rmin (minimum image), Maxwell-Boltzmann velocities at
