@@ -176,6 +176,10 @@ struct fcmd_system {
     // rebuilds or not, so it has nothing to overlap); FCMD_OVERLAP=0 or 1
     // in the environment decides instead
     int overlap = 1;
+    // every resort-th list rebuild of a trajectory re-sorts its atoms by
+    // cell; the others keep them in their slots (FCMD_RESORT, read by
+    // fcmd_create; 1 re-sorts at every rebuild)
+    int resort = 8;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *stage = nullptr;    // natom-sized staging (device)
     int *stage_i = nullptr;
@@ -302,7 +306,7 @@ int fill_params(fcmd_system *s, int k, const fcmd_params &p, TrajParams &t)
             if ((p.nh_mask >> a) & 1) nc += p.num[a];
         t.nkt = 3.0 * nc * CIKB * p.att;
     }
-    (void)s;
+    t.resort = s->resort;
     (void)k;
     return FCMD_OK;
 }
@@ -435,6 +439,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     }
     s->ntraj = ntraj;
     s->mixed = precision == FCMD_MIXED;
+    if (const char *rs = std::getenv("FCMD_RESORT")) s->resort = std::max(1, std::atoi(rs));
     s->list_width = list_width > 0 ? list_width : 128;
     s->hp.assign(params, params + ntraj);
     s->tp.resize(ntraj);
@@ -518,7 +523,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.wl_size, (long long)WL_LISTS * ntraj);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.cell_first, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, s->g.ncblk);
-    A(d.nn, N) A(d.posf, N);
+    A(d.nn, N) A(d.posf, N) A(d.cpos, N) A(d.cslot, N);
     if (s->g.list16) {
         A(d.list16, lbase);
     } else {

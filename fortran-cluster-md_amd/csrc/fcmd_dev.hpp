@@ -79,6 +79,7 @@ struct TrajParams {
     int hbase;                       // cluster-size histograms, natom+1 entries
     int nh_mask;                     // species bits coupled to Nose-Hoover
     int nh_refmass;                  // 1: m1 for every coupled atom (reference)
+    int resort;                      // every resort-th list rebuild re-sorts the atoms by cell
 };
 
 // work lists of the rebuild
@@ -122,6 +123,8 @@ struct TrajScalars {
     int ncx, ncy, ncz, pad0;
     int nh_iters;
     int pending;                     // v holds v_h: the NH kick (1+dth psi) is still to apply
+    int move;                        // this rebuild re-sorts the atoms (else: a sorted view)
+    int pad1;
 };
 
 // Device pointers of one engine.  Accelerations are double (FP64 path) or
@@ -165,6 +168,11 @@ struct DevPtrs {
     // current by the drift (the force kernel's positions).
     float4 *posf;
     int *cellid;                     // mixed path: the packed cell of posf.w, alone (the drift reads it)
+    // the list search's candidates in cell order: at a rebuild that keeps
+    // the atoms in their slots, cpos[k] = posf[cslot[k]] for the k-th atom of
+    // the cell order (a re-sorting rebuild reads posf itself, slot = k)
+    float4 *cpos;
+    int *cslot;
     // clusters: per slot
     ClusterScalars *cls;
     int *edeg, *edeg1, *edges, *parent, *parentf, *liq;

@@ -230,11 +230,18 @@ __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
             const int cz = cell_coord(z, icz, ncz);
             const int c = (cz * ncy + cy) * ncx + cx;
             const int slot = atomicAdd(d.cell_cnt + cbase + c, 1);
+            const int packed = cx | (cy << CELL_BITS) | (cz << (2 * CELL_BITS));
             d.x[i] = x;
             d.y[i] = y;
             d.z[i] = z;
-            d.atom_cell[i] = cx | (cy << CELL_BITS) | (cz << (2 * CELL_BITS));  // packed
+            d.atom_cell[i] = packed;
             d.atom_slot[i] = slot;
+            if (!ts.move) {
+                // the atom keeps its slot: its cell-offset record for the new
+                // grid here (a re-sorting rebuild writes it in k_permute)
+                d.posf[i] = make_posf(x, y, z, cx, cy, cz, ts);
+                if (d.cellid) d.cellid[i] = packed;
+            }
         }
     }
 }
@@ -333,9 +340,12 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
     }
 }
 
-// gather every per-atom array into cell order (into the alternate buffers)
-// and write each atom's cell-offset record for its cell (on the mixed path
-// also the cell alone, for the drift)
+// A re-sorting rebuild (ts.move) gathers every per-atom array into cell
+// order (into the alternate buffers) and writes each atom's cell-offset
+// record for its cell (on the mixed path also the cell alone, for the
+// drift).  Any other rebuild leaves the atoms in their slots and writes the
+// list search's view of them in cell order (cpos, cslot), 20 bytes an atom
+// instead of moving every array there and back.
 // the acceleration array of the precision in use: FP64, or FP32 on the mixed path
 template <typename AccT>
 __device__ __forceinline__ AccT *acc_array(double *p64, float *p32)
@@ -361,6 +371,13 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
         const int i = b * BLK + threadIdx.x;
         const bool act = i - tp.abase < tp.natom;
         const int o = act ? d.sorted[i] : i;
+        if (!ts.move) {
+            // atoms stay in their slots: the list search's candidates in
+            // cell order, position i of the order holding slot o
+            d.cpos[i] = d.posf[o];
+            d.cslot[i] = o;
+            return;
+        }
         const AccT *ax = acc_array<AccT>(d.ax, d.axf), *ay = acc_array<AccT>(d.ay, d.ayf),
                    *az = acc_array<AccT>(d.az, d.azf);
         const double x = d.x[o], y = d.y[o], z = d.z[o];
@@ -389,21 +406,26 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 }
 
 // ---------------------------------------------------------------------------
-// The reference's pair test on the permuted FP64 positions: on wrapped
-// positions, d = x(i) - x(j) with i the lower reference atom number,
+// The reference's pair test on the FP64 positions of the slots (after a
+// re-sorting rebuild the permuted arrays, else the arrays in place): on
+// wrapped positions, d = x(i) - x(j) with i the lower reference atom number,
 // d = d + L - L*int(1.5 + d/L), r2 <= atrskin^2 (neighbor-pbc.f:79-84).
 // b, ib: box sides and their inverses, read once by the caller (read here,
 // after the list stores, they would be reloaded for every candidate).
-__device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, int i, int j, double rs2, const double (&b)[3],
-                                                 const double (&ib)[3])
+struct RefArrays {
+    const double *x, *y, *z;
+    const int *oid;
+};
+__device__ __forceinline__ bool reference_pair_test(const RefArrays &r, int i, int j, double rs2,
+                                                   const double (&b)[3], const double (&ib)[3])
 {
     const double bx = b[0], by = b[1], bz = b[2];
     const double ibx = ib[0], iby = ib[1], ibz = ib[2];
-    const bool fwd = d.oid2[i] < d.oid2[j];
+    const bool fwd = r.oid[i] < r.oid[j];
     const int a = fwd ? i : j, c = fwd ? j : i;
-    double dx = d.x2[a] - d.x2[c];
-    double dy = d.y2[a] - d.y2[c];
-    double dz = d.z2[a] - d.z2[c];
+    double dx = r.x[a] - r.x[c];
+    double dy = r.y[a] - r.y[c];
+    double dz = r.z[a] - r.z[c];
     dx = dx + bx - bx * fint(1.5 + dx * ibx);
     dy = dy + by - by * fint(1.5 + dy * iby);
     dz = dz + bz - bz * fint(1.5 + dz * ibz);
@@ -434,7 +456,8 @@ __device__ __forceinline__ bool reference_pair_test(const DevPtrs &d, int i, int
 // takes the records' cells and, on such a side, the rounding image.
 template <int CU, bool SMALL, typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
-                                          int i, int &half, Sink sink)
+                                          int i, const float4 *cand, const int *cslot, const RefArrays &ref,
+                                          int &half, Sink sink)
 {
     const double rs2 = ts.rs2;
     const double bd[3] = {tp.box[0], tp.box[1], tp.box[2]}, ibd[3] = {tp.ibox[0], tp.ibox[1], tp.ibox[2]};
@@ -527,8 +550,11 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                 o = k >= T[2] ? off[3] : o;
                 o = g3 ? off[4] : o;
                 o = k >= T[4] ? off[5] : o;
+                // the candidate's record and slot (its position in the cell
+                // order, or the slot the view holds there)
                 jj[u] = k + o;
-                pj[u] = d.posf[jj[u]];
+                pj[u] = cand[jj[u]];
+                if (cslot) jj[u] = cslot[jj[u]];
                 moyf[u] = g3 ? -1.0f : (g1 ? 0.0f : 1.0f);  // rows oy = -1, 0, 1 (!SMALL)
             }
 #pragma unroll
@@ -552,7 +578,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                 }
                 const float r2f = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
                 bool in = r2f <= in2;
-                if (!in && r2f <= out2) in = reference_pair_test(d, i, j, rs2, bd, ibd);
+                if (!in && r2f <= out2) in = reference_pair_test(ref, i, j, rs2, bd, ibd);
                 if (in) {
                     sink(j, cnt);
                     ++cnt;
@@ -586,6 +612,12 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         const bool live = i - abase < tp.natom;
         const int width = tp.lwidth;
         int cnt = 0, half = 0;
+        // a re-sorting rebuild searches the permuted records and arrays
+        // (slot = position in the cell order); any other the view
+        const bool move = ts.move;
+        const float4 *cand = move ? d.posf : d.cpos;
+        const int *cslot = move ? nullptr : d.cslot;
+        const RefArrays ref = move ? RefArrays{d.x2, d.y2, d.z2, d.oid2} : RefArrays{d.x, d.y, d.z, d.oid};
         if (live) {
             LT *out = list_base<LT>(d) + list_slot(tp, i - abase);
             // plain (cached) stores: each lane appends at its own position, so
@@ -596,9 +628,9 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
                 if (k < width) out[(size_t)k * TILE] = (LT)(j - abase);
             };
             if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-                cnt = candidates<CU, true>(d, tp, ts, i, half, sink);
+                cnt = candidates<CU, true>(d, tp, ts, i, cand, cslot, ref, half, sink);
             else
-                cnt = candidates<CU, false>(d, tp, ts, i, half, sink);
+                cnt = candidates<CU, false>(d, tp, ts, i, cand, cslot, ref, half, sink);
         }
         if (cnt > width) {
             ts.err = ERR_LISTFULL;
@@ -614,6 +646,7 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
             if (red[0] > 0) atomicMax(&ts.nlist, red[0]);
             if (red[1]) atomicAdd(&ts.npairs, red[1]);
         }
+        if (!move) return;
         // copy back: every slot of the block takes its permuted arrays (no
         // thread of this kernel reads the primary ones); streamed
         // (nontemporal), and every load issued before any store

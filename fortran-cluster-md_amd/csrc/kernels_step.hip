@@ -79,6 +79,9 @@ __device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *ste
     ts.nlist = 0;
     ts.rebuild = 1;
     ts.nrebuild += 1;
+    // the first rebuild and every resort-th after it re-sort the atoms by
+    // cell (locality of the gathers); the others keep them in their slots
+    ts.move = (ts.nrebuild - 1) % tp.resort == 0;
 }
 
 // Fixed-order reduction of one trajectory's per-block partials by one block

@@ -113,3 +113,45 @@ def test_forces_beside_the_rebuild_give_the_same_bits(use_graph, precision, monk
         for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
             np.testing.assert_array_equal(a[k], b[k])
         assert (ea, pa, ra) == (eb, pb, rb)
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+def test_resort_interval_keeps_the_atoms_bits(precision, monkeypatch):
+    """Rebuilds that keep the atoms in their slots (a sorted view for the
+    list search) and rebuilds that re-sort them by cell build the same
+    lists in the same order (cells z-major, reference numbers within a
+    cell), so every per-atom quantity is the same bits whatever the re-sort
+    interval.  Velocity Verlet with the adaptive skin: no sum over atoms
+    feeds back into the motion (the rebuild rule takes a maximum), so whole
+    trajectories agree bit for bit, and so do the reference-layout half
+    lists, the cluster labels and the largest-cluster series; the energies
+    (sums in slot order) to rounding."""
+    params, states = _small_ensemble()
+    for p in params:
+        p.integrator, p.ethst_nh = fcmd.INT_MAV, 0
+    res = []
+    for resort in ("1", "3", "8"):
+        monkeypatch.setenv("FCMD_RESORT", resort)
+        with fcmd.System(params, precision) as s:
+            W.setup(s, params, states)
+            series, _ = s.run(90, cluster_every=30, use_graph=True)
+            st = [s.download(t) for t in range(len(params))]
+            sc = [s.scalars(t) for t in range(len(params))]
+            hl = [s.halflist(t) for t in (0, 5, 11)]
+            cl = [s.stoddard(90, traj=t)["cl"] for t in (0, 5, 11)]
+        res.append((st, sc, hl, cl, series))
+    monkeypatch.delenv("FCMD_RESORT", raising=False)
+    assert max(sc.nrebuild for sc in res[0][1]) >= 4  # several rebuilds, so both kinds ran
+    for st, sc, hl, cl, series in res[1:]:
+        for a, b in zip(res[0][0], st):
+            for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+                np.testing.assert_array_equal(a[k], b[k])
+        for a, b in zip(res[0][1], sc):
+            assert (a.nrebuild, a.npairs, a.nlist, a.atrskin) == (b.nrebuild, b.npairs, b.nlist, b.atrskin)
+            assert b.epot == pytest.approx(a.epot, rel=1e-12)
+        for (la, ia), (lb, ib) in zip(res[0][2], hl):
+            np.testing.assert_array_equal(la, lb)
+            np.testing.assert_array_equal(ia, ib)
+        for a, b in zip(res[0][3], cl):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(res[0][4], series)
