@@ -82,3 +82,48 @@ def test_device_statistics_match_snapshot_histograms(nmax, warmup):
     # the two runs are the same run
     for t in range(len(cs)):
         np.testing.assert_array_equal(a.download(t)["x"], b.download(t)["x"])
+
+
+def test_device_statistics_match_the_oracle_run():
+    """The same statistics against the oracle: each trajectory integrated by
+    the C restatement of the reference (int-verlet-nh-mav.f with
+    neighbor-pbc-mav.f, for-a-lj-n-pbc-m.f) from the same state, its
+    Stillinger analysis (stoddard-pbc-n-m.f) taken before the steps the
+    device analyses (main.f:133-164 order).  The largest-cluster series, the
+    first passage to 8 atoms (helpers/passage-time.f:55-75: the first
+    analysis at which the largest cluster reaches the size) and the sizer
+    counts (helpers/sizer.f:59-103) must agree exactly."""
+    import pyoracle as po
+
+    cs, build = _systems()
+    a = build()
+    step0 = a.scalars(0).step
+    nmax, warmup = 6, 100
+    a.size_counts(nmax, warmup=step0 + warmup)
+    series, fpt = a.run(EVERY * NSNAP, cluster_every=EVERY, fpt_size=8)
+    for t, c in enumerate(cs):
+        o = po.Oracle(c, "nose")
+        o.neighbor(0)
+        o.accatom()
+        snaps, largest = [], []
+        for k in range(NSNAP):
+            st = step0 + k * EVERY
+            o.stoddard(st)
+            snaps.append((st, o.arr["csi"].copy()))
+            h = o.arr["csi"][1:]
+            largest.append(int(np.nonzero(h)[0].max()) + 1 if h.any() else 0)
+            for i in range(st, st + EVERY):
+                o.integrator(i)
+        assert list(series[:, t]) == largest, t
+        assert max(largest) > 1
+        first = next((ts for (ts, _), g in zip(snaps, largest) if g >= 8), -1)
+        assert fpt[t] == first, t
+        want, stopped = _sizer(snaps, nmax, step0 + warmup)
+        got, got_stopped = a.get_size_counts(t)
+        np.testing.assert_array_equal(got[1:], want[1:])
+        assert got_stopped == stopped
+        # and the trajectories themselves still agree (summation order only)
+        got_x = a.download(t)["x"]
+        L = c["box"][0]
+        d = np.abs(got_x - o.arr["x"])
+        assert np.minimum(d, np.abs(d - L)).max() < 1e-6 * L
