@@ -38,7 +38,8 @@ Reported:
                    reported apart)
   dtype            f32+f64: pair arithmetic in FP32, state, integrator and
                    reductions in FP64 (the fp64_* fields are FP64 throughout)
-  tuning           launch tunables the autotuner picked during the warmup
+  tuning           launch tunables the autotuner picked during the warmup (or
+                   those --tuning fixed)
 
 Usage: python bench.py [--steps K] [--warmup W] [--workload NAME] [--gpus 1]
 """
@@ -257,6 +258,9 @@ def main():
                     help="launch every step's kernels on the stream instead of replaying a graph")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (262k FP64, 2M FP64 vs mixed, mixture)")
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch tunables force_u,nbr_cu,wl_grid instead of the autotuner "
+                         "(for profiles whose passes must run the same kernels)")
     ap.add_argument("--nbtim", type=int, default=0,
                     help="override the deck's neighbour update time (nbtim, the adaptive skin's "
                          "target rebuild interval); 0 keeps the workload's")
@@ -287,7 +291,13 @@ def main():
         s = build_system(params, states, prec)
         # warmup with an analysis: every kernel loaded, the graph captured
         s.run(Wm, cluster_every=every, fpt_size=fpt_size, use_graph=use_graph)
-        tuning = s.autotune(rounds=5)  # 30 more untimed steps; recaptures the graph
+        if args.tuning:
+            fu, nc, wg = (int(v) for v in args.tuning.split(","))
+            s.set_tuning(fu, nc, wg)  # recaptures the graph
+            s.run(30, cluster_every=0, use_graph=use_graph)  # the steps the autotuner would take
+            tuning = s.tuning()
+        else:
+            tuning = s.autotune(rounds=5)  # 30 more untimed steps; recaptures the graph
         log(f"[bench] {name}: tuning {tuning}")
         h0 = hamiltonian(s, params)
         nr0 = np.array([s.scalars(t).nrebuild for t in range(len(params))])
