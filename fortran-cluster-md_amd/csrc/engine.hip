@@ -160,7 +160,7 @@ void launch_neighbor_entry(const DevPtrs &d, int traj, int ntraj, int step, int 
 struct fcmd_system {
     int ntraj = 0;
     bool mixed = false;
-    int list_width = 128;
+    int list_width = 256;
     std::vector<fcmd_params> hp;
     std::vector<TrajParams> tp;
     LaunchDims g{};
@@ -440,7 +440,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     s->ntraj = ntraj;
     s->mixed = precision == FCMD_MIXED;
     if (const char *rs = std::getenv("FCMD_RESORT")) s->resort = std::max(1, std::atoi(rs));
-    s->list_width = list_width > 0 ? list_width : 128;
+    s->list_width = list_width > 0 ? list_width : 256;
     s->hp.assign(params, params + ntraj);
     s->tp.resize(ntraj);
     int rc;

@@ -96,7 +96,8 @@ const char *fcmd_error_text(int code);
 /* Allocate an ensemble of ntraj trajectories on the current HIP device.
  * precision: FCMD_FP64 or FCMD_MIXED.  list_width: neighbour-list slots per
  * atom, i.e. the most neighbours (within atrskin, both directions) any atom
- * may have; 0 selects 128.  It takes the role of const.inc mxnlist: a list
+ * may have; 0 selects 256 (a liquid droplet at the adaptive skin's usual
+ * 2.8-3.3 sigma holds 75-130).  It takes the role of const.inc mxnlist: a list
  * rebuild that finds an atom with more neighbours fails with
  * FCMD_ERR_LISTFULL, as the reference stops when its list array is nearly
  * full (neighbor-pbc.f:89-93).  Memory: 2 * list_width bytes per atom when
