@@ -523,6 +523,7 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.wl_size, (long long)WL_LISTS * ntraj);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.cell_first, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, s->g.ncblk);
+    A(d.big_cnt, 1) A(d.big_cells, N / (CS_REG + 1) + 1);
     A(d.nn, N) A(d.posf, N) A(d.cpos, N) A(d.cslot, N);
     if (s->g.list16) {
         A(d.list16, lbase);

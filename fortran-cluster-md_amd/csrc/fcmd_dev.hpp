@@ -85,6 +85,10 @@ struct TrajParams {
 // work lists of the rebuild
 enum { WL_BLK = 0, WL_CCHK = 1, WL_TRAJ = 2, WL_LISTS = 3 };
 
+// the cell sort: a cell of up to CS_REG atoms is sorted by one thread, a
+// larger one (a liquid droplet's cells hold 15-30) by a wavefront
+constexpr int CS_REG = 8;
+
 // first list slot of trajectory-relative atom slot rel; its k-th neighbour is
 // at + k*TILE (the 64 lanes of a tile read 256 contiguous bytes per slot)
 __host__ __device__ inline long long list_slot(const TrajParams &tp, int rel)
@@ -153,6 +157,9 @@ struct DevPtrs {
     // cells
     int *cell_cnt, *cell_start, *sorted;
     int *cell_first;                 // final trajectory-relative cell starts (cell sort -> list search)
+    int *big_cnt;                    // the cells of more than CS_REG atoms the cell sort left, and
+    int2 *big_cells;                 // each one's (first entry of sorted, atom count); room for
+                                     // natoms / (CS_REG + 1) + 1
     int *atom_cell, *atom_slot;      // per slot at binning: packed cell (CELL_BITS each), slot in cell
     long long *scan_tmp_ll;          // chunk totals for segmented scans
     // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE],

@@ -95,6 +95,7 @@ __global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
             d.wl_cnt[k] = lds[0][WL_LISTS + k] + lds[1][WL_LISTS + k] + lds[2][WL_LISTS + k] +
                           lds[3][WL_LISTS + k];
     }
+    if (t == 0 && threadIdx.x == 0) d.big_cnt[0] = 0;  // the cell sort's list of large cells
     if (!rebuilding(d.ts[t])) return;
     const TrajParams &tp = d.tp[t];
     for (int q = threadIdx.x; q < tp.nblk; q += BLK) d.wl_blk[off[WL_BLK] + q] = tp.bfirst + q;
@@ -273,9 +274,6 @@ __device__ __forceinline__ void cmp_swap(int &ka, int &va, int &kb, int &vb)
     }
 }
 
-// cells of up to this many atoms are sorted in registers
-constexpr int CS_REG = 8;
-
 // sort of each cell's atoms by reference atom number, so the order (and every
 // later floating-point sum) does not depend on atomics; also writes each
 // cell's final start to cell_first.  One thread per cell: a chunk's
@@ -284,7 +282,9 @@ constexpr int CS_REG = 8;
 // runs during the rebuild), and with the final starts in an array of their
 // own no thread waits for another.  A cell of up to CS_REG atoms loads its
 // members and their numbers at once and sorts them in registers (odd-even
-// transposition, CS_REG phases); a larger one is sorted in place.
+// transposition, CS_REG phases); a larger one goes to a list that k_bigsort
+// sorts a wavefront a cell (a thread sorting 15-30 atoms in place, a chain of
+// dependent loads per step, held the whole launch once droplets had formed).
 constexpr int CS_PARTS = SCAN_CHUNK / BLK;  // workgroups per chunk
 
 __device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
@@ -336,7 +336,37 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
                            ? d.cell_start[cell + 1]
                            : (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
         d.cell_first[cell] = s0 + add;
-        sort_cell(d, d.sorted + abase + s0 + add, e0 - s0);
+        const int n = e0 - s0;
+        if (n > CS_REG)
+            d.big_cells[atomicAdd(d.big_cnt, 1)] = make_int2(abase + s0 + add, n);
+        else
+            sort_cell(d, d.sorted + abase + s0 + add, n);
+    }
+}
+
+// The large cells, a wavefront each: lane L holds member L and its reference
+// number, counts the numbers below its own over the cell (one shuffle a
+// member) and stores the member at that rank.  The numbers are distinct, so
+// this is the order the one-thread sort gives.  A cell of more than 64 atoms
+// (never met at liquid density and the list radii here) falls back to lane 0.
+__global__ __launch_bounds__(BLK) void k_bigsort(DevPtrs d)
+{
+    const int lane = threadIdx.x & 63;
+    const int nw = (int)(gridDim.x * (BLK / 64));
+    const int n_big = d.big_cnt[0];
+    for (int c = (int)(blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)); c < n_big; c += nw) {
+        const int2 bc = d.big_cells[c];
+        int *a = d.sorted + bc.x;
+        const int n = bc.y;
+        if (n > 64) {
+            if (lane == 0) sort_cell(d, a, n);
+            continue;
+        }
+        const int v = lane < n ? a[lane] : 0;
+        const int key = lane < n ? d.oid[v] : INT_MAX;
+        int rank = 0;
+        for (int m = 0; m < n; ++m) rank += __shfl(key, m, 64) < key;
+        if (lane < n) a[rank] = v;
     }
 }
 
@@ -695,6 +725,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, gall, blk, 0, s, d);
     hipLaunchKernelGGL(k_cellsort, dim3(xcd_grid(std::min(CS_PARTS * g.ncblk, wg))), blk, 0, s, d);
+    hipLaunchKernelGGL(k_bigsort, dim3(1024), blk, 0, s, d);
     if (mixed)
         hipLaunchKernelGGL(k_permute<float>, gall, blk, 0, s, d);
     else
