@@ -319,7 +319,7 @@ int upload_params(fcmd_system *s)
 
 int check_traj(fcmd_system *s, int traj) { return (s && traj >= 0 && traj < s->ntraj) ? FCMD_OK : FCMD_ERR_ARG; }
 
-int enqueue_step(fcmd_system *s);
+int enqueue_step(fcmd_system *s, bool energy = true);
 
 int sync_err(fcmd_system *s)
 {
@@ -350,7 +350,9 @@ int capture_graph(fcmd_system *s, int gsteps)
     hipGraph_t gr;
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = FCMD_OK;
-    for (int k = 0; k < gsteps && !rc; ++k) rc = enqueue_step(s);
+    // the energy and virial of the graph's last step only: its scalars are the
+    // ones a run can end with
+    for (int k = 0; k < gsteps && !rc; ++k) rc = enqueue_step(s, k == gsteps - 1);
     HIPCHK(hipStreamEndCapture(s->stream, &gr));
     if (rc) {
         (void)hipGraphDestroy(gr);
@@ -373,22 +375,27 @@ int capture_graph(fcmd_system *s, int gsteps)
 // trajectories' forces follow it.  Every atom block is computed by the same
 // kernel code either way, so the results are the same bits as the forces of
 // all blocks in one launch after the rebuild (s->overlap = 0).
-int enqueue_step(fcmd_system *s)
+//
+// energy: the step also forms epot, the virial and the pressure.  A run
+// leaves them out of every step whose scalars the next step overwrites
+// before anyone can read them; positions, velocities, forces and every other
+// scalar are the same bits either way.
+int enqueue_step(fcmd_system *s, bool energy)
 {
     launch_drift(s->d, s->g, s->mixed, s->stream);
     if (!s->overlap) {
         launch_rebuild(s->d, s->g, s->mixed, s->stream);
-        launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, energy);
         return FCMD_OK;
     }
     HIPCHK(hipEventRecord(s->ev_fork, s->stream));
     HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->side, {}, FP_KEPT);
+    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->side, {}, FP_KEPT, energy);
     HIPCHK(hipEventRecord(s->ev_join, s->side));
     launch_rebuild(s->d, s->g, s->mixed, s->stream);
-    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_REBUILT);
+    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_REBUILT, energy);
     HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
-    launch_traj_post(s->d, s->g, FORCE_STEP, s->stream);
+    launch_traj_post(s->d, s->g, FORCE_STEP, s->stream, {}, energy);
     return FCMD_OK;
 }
 
@@ -923,7 +930,7 @@ int fcmd_run(fcmd_system *s, int nsteps, int cluster_every, int fpt_size, int *s
         if (use_graph && s->graph)
             for (; k + s->graph_steps <= todo; k += s->graph_steps) HIPCHK(hipGraphLaunch(s->graph, s->stream));
         for (; k < todo; ++k)
-            if (int rc = enqueue_step(s)) return rc;
+            if (int rc = enqueue_step(s, done + k + 1 == nsteps)) return rc;
         HIPCHK(hipGetLastError());
         done += todo;
     }
@@ -977,7 +984,7 @@ int fcmd_run_noalloc(fcmd_system *s, int nsteps, int cluster_every, int fpt_size
         }
         const int todo = std::min(nsteps - done, nsnap > 0 ? cluster_every - done % cluster_every : nsteps - done);
         for (int k = 0; k < todo; ++k)
-            if (int rc = enqueue_step(s)) return rc;
+            if (int rc = enqueue_step(s, done + k + 1 == nsteps)) return rc;
         HIPCHK(hipGetLastError());
         done += todo;
     }
@@ -1095,9 +1102,11 @@ int fcmd_autotune(fcmd_system *s, int rounds, int *tuning)
         HIPCHK(hipEventRecord(e[0], s->stream));
         launch_rebuild(s->d, g, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[1], s->stream));
-        launch_force_atoms(s->d, g, s->mixed, s->single_species, FORCE_STEP, s->stream);
+        // the run's kernels: the energy on the last step only
+        const bool en = k == nsteps - 1;
+        launch_force_atoms(s->d, g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_ALL, en);
         HIPCHK(hipEventRecord(e[2], s->stream));
-        launch_traj_post(s->d, g, FORCE_STEP, s->stream);
+        launch_traj_post(s->d, g, FORCE_STEP, s->stream, {}, en);
         HIPCHK(hipEventRecord(e[3], s->stream));
     }
     HIPCHK(hipGetLastError());
@@ -1144,8 +1153,10 @@ int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
         launch_drift_atoms(s->d, s->g, s->mixed, s->stream, kt[0]);
         launch_traj_pre(s->d, s->g, s->stream, kt[1]);
         launch_rebuild(s->d, s->g, s->mixed, s->stream, kt[2]);
-        launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, kt[3]);
-        launch_traj_post(s->d, s->g, FORCE_STEP, s->stream, kt[4]);
+        // as a run's captured graph does: the energy every 10th step
+        const bool en = (k + 1) % 10 == 0 || k == nsteps - 1;
+        launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, kt[3], FP_ALL, en);
+        launch_traj_post(s->d, s->g, FORCE_STEP, s->stream, kt[4], en);
     }
     // the run loop defers the NH update to the next drift; only the last step
     // of a run (or an analysis) pays it, so it is timed once

@@ -45,15 +45,20 @@ inline void launch_timed(F kern, dim3 grid, dim3 blk, hipStream_t s, const KTime
 }
 
 // launch_drift = launch_drift_atoms (k_drift) + launch_traj_pre (k_traj_pre);
-// launch_force = launch_force_atoms (k_force) + launch_traj_post (k_traj_post)
+// launch_force = launch_force_atoms (k_force) + launch_traj_post (k_traj_post).
+// energy: the force launch also sums the potential energy and the virial, and
+// k_traj_post sets epot, virial and pressure from them; a run step whose
+// scalars nobody can read before the next step overwrites them (every step of
+// a run but its last and the last of each captured graph) leaves them out.
 void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt = {});
 void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s, const KTimer &kt = {});
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                  hipStream_t s);
+                  hipStream_t s, bool energy = true);
 void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                        hipStream_t s, const KTimer &kt = {}, int part = FP_ALL);
-void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt = {});
+                        hipStream_t s, const KTimer &kt = {}, int part = FP_ALL, bool energy = true);
+void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt = {},
+                      bool energy = true);
 // applies any pending Nose-Hoover final update and clears the flags
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt = {});

@@ -318,20 +318,23 @@ __device__ void nh_solve(const TrajParams &tp, TrajScalars &ts, double S)
 // thermostat (species-selective coupling only); wsum: per-atom pair virial
 // terms over the full list.
 __device__ void traj_post(const TrajParams &tp, TrajScalars &ts, double esum, double eks, double ekfree,
-                          double wsum, int mode)
+                          double wsum, int mode, bool energy)
 {
-    // every pair is seen from both atoms of the full list
-    esum = 0.5 * esum;
-    double w = 0.5 * wsum;
-    if (tp.accatom == ACC_1) {
-        ts.epot = esum * tp.eps1 * 4.0;
-        w = w * 48.0 * tp.eps1;  // for-a-lj-n-pbc.f:114
-    } else {
-        ts.epot = esum;
+    if (energy) {
+        // every pair is seen from both atoms of the full list
+        esum = 0.5 * esum;
+        double w = 0.5 * wsum;
+        if (tp.accatom == ACC_1) {
+            ts.epot = esum * tp.eps1 * 4.0;
+            w = w * 48.0 * tp.eps1;  // for-a-lj-n-pbc.f:114
+        } else {
+            ts.epot = esum;
+        }
+        // for-a-lj-n-pbc.f:116-117: 1e13/V (N k T - W/3), T the target temperature
+        ts.virial = w;
+        ts.pressure =
+            (1.0e13 / (tp.box[0] * tp.box[1] * tp.box[2])) * ((double)tp.natom * CIKB * tp.att - w / 3.0);
     }
-    // for-a-lj-n-pbc.f:116-117: 1e13/V (N k T - W/3), T the target temperature
-    ts.virial = w;
-    ts.pressure = (1.0e13 / (tp.box[0] * tp.box[1] * tp.box[2])) * ((double)tp.natom * CIKB * tp.att - w / 3.0);
     if (mode == FORCE_ACCATOM) return;
     if (tp.integrator == INT_NH) {
         nh_solve(tp, ts, eks);
@@ -344,7 +347,7 @@ __device__ void traj_post(const TrajParams &tp, TrajScalars &ts, double esum, do
 }
 
 // one block per trajectory, after the forces
-__global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
+__global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode, int energy)
 {
     __shared__ double lds[4];
     const int t = blockIdx.x;
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode)
     if (ts.err) return;
     double esum, eks, ekfree, wsum;
     reduce_partials(tp, d.part, P_EPOT, P_EKS, P_SUMV2, false, P_VIR, lds, esum, eks, ekfree, wsum);
-    if (threadIdx.x == 0) traj_post(tp, ts, esum, eks, ekfree, wsum, mode);
+    if (threadIdx.x == 0) traj_post(tp, ts, esum, eks, ekfree, wsum, mode, energy != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -463,7 +466,7 @@ __device__ __forceinline__ void walk_list(unsigned (*stage)[BLK], const H &h, Lo
     }
 }
 
-template <int FORM, int U, typename H>
+template <int FORM, bool EN, int U, typename H>
 __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                               int i, const H &h, int ti, double &fx, double &fy, double &fz,
                                               double &e, double &w, unsigned (*stage)[BLK])
@@ -508,14 +511,18 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
                     // (for-a-lj-n-pbc.f:106) share
                     const double q = tp.c2[0] * rij6 - tp.c3[0];
                     fij = rij2 * rij6 * q;
-                    e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
-                    w += rij6 * q;
+                    if (EN) {
+                        e += rij6 * (tp.c2[0] * rij6 - tp.c1[0]);
+                        w += rij6 * q;
+                    }
                 } else {
                     const int pij = ti * MAXS + tj[u];
                     const double q = tp.c2[pij] * rij6 - tp.c3[pij];
                     fij = tp.e48[pij] * rij2 * rij6 * q;
-                    e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
-                    w += tp.e48[pij] * rij6 * q;
+                    if (EN) {
+                        e += tp.e4[pij] * rij6 * (tp.c2[pij] * rij6 - tp.c1[pij]);
+                        w += tp.e48[pij] * rij6 * q;
+                    }
                 }
                 fx = fx + fij * dx;
                 fy = fy + fij * dy;
@@ -554,7 +561,7 @@ __device__ __forceinline__ bool reference_cut_test(const DevPtrs &d, const TrajP
     return dx * dx + dy * dy + dz * dz <= rc2;
 }
 
-template <int FORM, bool SMALL, int U, typename H>
+template <int FORM, bool EN, bool SMALL, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
                                                double &fy, double &fz, double &e, double &w,
@@ -614,15 +621,20 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
             const float s6 = u2 * u2 * u2;
             const float t = s6 - 0.5f;
             fij = ri2 * s6 * t;
-            ef = __builtin_fmaf(sgn * s6, s6 - 1.0f, ef);
-            wf = __builtin_fmaf(sgn * s6, t, wf);
+            if (EN) {
+                ef = __builtin_fmaf(sgn * s6, s6 - 1.0f, ef);
+                wf = __builtin_fmaf(sgn * s6, t, wf);
+            }
         } else {
             const float r6 = ri2 * ri2 * ri2;
             const int pij = ti * MAXS + tjj;
             const float q = __builtin_fmaf(tp.c2f[pij], r6, -tp.c3f[pij]);
             fij = (float)tp.e48[pij] * ri2 * r6 * q;
-            ef = __builtin_fmaf(sgn * (float)tp.e4[pij], r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]), ef);
-            wf = __builtin_fmaf(sgn * fij, r2, wf);
+            if (EN) {
+                ef = __builtin_fmaf(sgn * (float)tp.e4[pij], r6 * __builtin_fmaf(tp.c2f[pij], r6, -tp.c1f[pij]),
+                                    ef);
+                wf = __builtin_fmaf(sgn * fij, r2, wf);
+            }
         }
         const double fd = (double)(sgn * fij);
         sfx = __builtin_fma((double)dxf, fd, sfx);
@@ -671,7 +683,7 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     w += (double)wf;
 }
 
-template <typename AccT, int FORM, int U, typename LT>
+template <typename AccT, int FORM, bool EN, int U, typename LT>
 __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                          int i, int rel, int ti, double &fx, double &fy, double &fz,
                                          double &e, double &w, unsigned (*stage)[BLK])
@@ -681,11 +693,11 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
     if constexpr (sizeof(AccT) == 4) {
         const float4 pi = d.posf[i];
         if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-            pair_sum_mixed<FORM, true, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
+            pair_sum_mixed<FORM, EN, true, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
         else
-            pair_sum_mixed<FORM, false, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
+            pair_sum_mixed<FORM, EN, false, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
     } else
-        pair_sum_fp64<FORM, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, w, stage);
+        pair_sum_fp64<FORM, EN, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, w, stage);
 }
 
 // LJ forces over the full neighbour list plus the integrator's fused
@@ -696,11 +708,13 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
 // batch are held to 72 VGPRs (7 wavefronts a SIMD; without spills), which
 // the 16-bit list decode would otherwise just exceed.
 //
+// EN: the energy and virial sums are formed (see launch_force).
+//
 // part: which atom blocks the launch takes.  FP_ALL: every block.
 // FP_KEPT: the blocks of trajectories that keep their list this step (this
 // launch can run beside the rebuild of the others).  FP_REBUILT: the blocks
 // of the trajectories that rebuild, from the rebuild's work list (after it).
-template <typename AccT, bool MULTI, int U, typename LT>
+template <typename AccT, bool MULTI, int U, typename LT, bool EN>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U <= 4 ? 7 : 1))) void k_force(
     DevPtrs d, int mode, int part)
 {
@@ -734,9 +748,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
         // while it runs
         double fx, fy, fz;
         if (form1)
-            pair_sum<AccT, ACC_1, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+            pair_sum<AccT, ACC_1, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
         else
-            pair_sum<AccT, ACC_M, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+            pair_sum<AccT, ACC_M, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
         // the velocities after the pair sum: loaded before it, they would
         // hold six registers through the pair loop, which the single-species
         // kernel (held to 72) would spill
@@ -799,13 +813,22 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
             }
         }
     }
-    double r[4] = {e, eks, ekfree, w};
-    block_reduce_end<4>(r, {false, false, false, false}, lds);
-    if (threadIdx.x == 0) {
-        d.part[(size_t)b * NPART + P_EPOT] = r[0];
-        d.part[(size_t)b * NPART + P_EKS] = r[1];
-        d.part[(size_t)b * NPART + P_SUMV2] = r[2];
-        d.part[(size_t)b * NPART + P_VIR] = r[3];
+    if constexpr (EN) {
+        double r[4] = {e, eks, ekfree, w};
+        block_reduce_end<4>(r, {false, false, false, false}, lds);
+        if (threadIdx.x == 0) {
+            d.part[(size_t)b * NPART + P_EPOT] = r[0];
+            d.part[(size_t)b * NPART + P_EKS] = r[1];
+            d.part[(size_t)b * NPART + P_SUMV2] = r[2];
+            d.part[(size_t)b * NPART + P_VIR] = r[3];
+        }
+    } else {
+        double r[2] = {eks, ekfree};
+        block_reduce_end<2>(r, {false, false}, lds);
+        if (threadIdx.x == 0) {
+            d.part[(size_t)b * NPART + P_EKS] = r[0];
+            d.part[(size_t)b * NPART + P_SUMV2] = r[1];
+        }
     }
 }
 
@@ -891,55 +914,65 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
     launch_traj_pre(d, g, s);
 }
 
-template <typename AccT, bool MULTI, typename LT>
+template <typename AccT, bool MULTI, typename LT, bool EN>
 static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
                             const KTimer &kt)
 {
     const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
     if (g.force_u == 2)
-        launch_timed(k_force<AccT, MULTI, 2, LT>, grid, blk, s, kt, d, mode, part);
+        launch_timed(k_force<AccT, MULTI, 2, LT, EN>, grid, blk, s, kt, d, mode, part);
     else if (g.force_u == 8)
-        launch_timed(k_force<AccT, MULTI, 8, LT>, grid, blk, s, kt, d, mode, part);
+        launch_timed(k_force<AccT, MULTI, 8, LT, EN>, grid, blk, s, kt, d, mode, part);
     else
-        launch_timed(k_force<AccT, MULTI, 4, LT>, grid, blk, s, kt, d, mode, part);
+        launch_timed(k_force<AccT, MULTI, 4, LT, EN>, grid, blk, s, kt, d, mode, part);
 }
 
 template <typename AccT, bool MULTI>
 static void launch_force_u(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
-                           const KTimer &kt)
+                           const KTimer &kt, bool energy)
 {
-    if (g.list16)
-        launch_force_lt<AccT, MULTI, unsigned short>(d, g, mode, part, s, kt);
-    else
-        launch_force_lt<AccT, MULTI, int>(d, g, mode, part, s, kt);
-}
-
-void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                        hipStream_t s, const KTimer &kt, int part)
-{
-    if (mixed) {
-        if (single_species)
-            launch_force_u<float, false>(d, g, mode, part, s, kt);
+    if (g.list16) {
+        if (energy)
+            launch_force_lt<AccT, MULTI, unsigned short, true>(d, g, mode, part, s, kt);
         else
-            launch_force_u<float, true>(d, g, mode, part, s, kt);
+            launch_force_lt<AccT, MULTI, unsigned short, false>(d, g, mode, part, s, kt);
     } else {
-        if (single_species)
-            launch_force_u<double, false>(d, g, mode, part, s, kt);
+        if (energy)
+            launch_force_lt<AccT, MULTI, int, true>(d, g, mode, part, s, kt);
         else
-            launch_force_u<double, true>(d, g, mode, part, s, kt);
+            launch_force_lt<AccT, MULTI, int, false>(d, g, mode, part, s, kt);
     }
 }
 
-void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt)
+void launch_force_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
+                        hipStream_t s, const KTimer &kt, int part, bool energy)
 {
-    launch_timed(k_traj_post, dim3(g.ntraj), dim3(BLK), s, kt, d, mode);
+    if (mode == FORCE_ACCATOM) energy = true;  // AccAtom() always sets epot
+    if (mixed) {
+        if (single_species)
+            launch_force_u<float, false>(d, g, mode, part, s, kt, energy);
+        else
+            launch_force_u<float, true>(d, g, mode, part, s, kt, energy);
+    } else {
+        if (single_species)
+            launch_force_u<double, false>(d, g, mode, part, s, kt, energy);
+        else
+            launch_force_u<double, true>(d, g, mode, part, s, kt, energy);
+    }
+}
+
+void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream_t s, const KTimer &kt,
+                      bool energy)
+{
+    if (mode == FORCE_ACCATOM) energy = true;
+    launch_timed(k_traj_post, dim3(g.ntraj), dim3(BLK), s, kt, d, mode, (int)energy);
 }
 
 void launch_force(const DevPtrs &d, const LaunchDims &g, bool mixed, bool single_species, int mode,
-                  hipStream_t s)
+                  hipStream_t s, bool energy)
 {
-    launch_force_atoms(d, g, mixed, single_species, mode, s);
-    launch_traj_post(d, g, mode, s);
+    launch_force_atoms(d, g, mixed, single_species, mode, s, {}, FP_ALL, energy);
+    launch_traj_post(d, g, mode, s, {}, energy);
 }
 
 __global__ void k_clear_pending(DevPtrs d, int ntraj)

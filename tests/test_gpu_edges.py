@@ -134,3 +134,23 @@ def test_nose_hoover_without_a_meaningful_update_is_reported():
     with pytest.raises(fcmd.FcmdError, match="did not converge") as e:
         s.integrator(2)
     assert e.value.code == 3
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+@pytest.mark.parametrize("nsteps,use_graph", [(40, True), (37, True), (23, False)])
+def test_run_ends_with_its_last_steps_energy(precision, nsteps, use_graph):
+    """A run forms epot, the virial and the pressure only where they can be
+    read: on its last step (and on the last step of each captured graph).  So
+    after any run they must equal AccAtom()'s on the final state, bit for bit,
+    whether the run ended on a graph replay or on single steps."""
+    from fcmd import workloads as W
+    params, states, meta = W.ensemble512(ntraj=6)
+    with fcmd.System(params, precision) as s:
+        W.setup(s, params, states)
+        s.run(nsteps, cluster_every=0, use_graph=use_graph)
+        got = [(x.epot, x.virial, x.pressure, x.step) for x in (s.scalars(t) for t in range(len(params)))]
+        s.accatom()
+        for t, (e, w, p, step) in enumerate(got):
+            sc = s.scalars(t)
+            assert sc.step == step
+            assert (sc.epot, sc.virial, sc.pressure) == (e, w, p), t
