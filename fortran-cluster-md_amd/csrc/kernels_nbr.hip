@@ -486,7 +486,8 @@ __device__ __forceinline__ bool reference_pair_test(const RefArrays &r, int i, i
 // takes the records' cells and, on such a side, the rounding image.
 template <int CU, bool SMALL, typename Sink>
 __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
-                                          int i, const float4 *cand, const int *cslot, const RefArrays &ref,
+                                          int i, const float4 *cand, bool view, const int *cslot,
+                                          const RefArrays &ref,
                                           int &half, Sink sink)
 {
     const double rs2 = ts.rs2;
@@ -584,7 +585,7 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
                 // order, or the slot the view holds there)
                 jj[u] = k + o;
                 pj[u] = cand[jj[u]];
-                if (cslot) jj[u] = cslot[jj[u]];
+                if (view) jj[u] = cslot[jj[u]];
                 moyf[u] = g3 ? -1.0f : (g1 ? 0.0f : 1.0f);  // rows oy = -1, 0, 1 (!SMALL)
             }
 #pragma unroll
@@ -646,21 +647,24 @@ __global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
         // (slot = position in the cell order); any other the view
         const bool move = ts.move;
         const float4 *cand = move ? d.posf : d.cpos;
-        const int *cslot = move ? nullptr : d.cslot;
         const RefArrays ref = move ? RefArrays{d.x2, d.y2, d.z2, d.oid2} : RefArrays{d.x, d.y, d.z, d.oid};
         if (live) {
             LT *out = list_base<LT>(d) + list_slot(tp, i - abase);
             // plain (cached) stores: each lane appends at its own position, so
             // one store instruction covers parts of many list rows, and the L2
             // merges those pieces into whole lines (nontemporal stores sent the
-            // pieces on one by one: the rebuild took 10 % longer)
+            // pieces on one by one: the rebuild took 10 % longer).  The store
+            // address advances a row per entry (sink is called with k = 0, 1, ...)
             auto sink = [&](int j, int k) {
-                if (k < width) out[(size_t)k * TILE] = (LT)(j - abase);
+                if (k < width) {
+                    *out = (LT)(j - abase);
+                    out += TILE;
+                }
             };
             if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-                cnt = candidates<CU, true>(d, tp, ts, i, cand, cslot, ref, half, sink);
+                cnt = candidates<CU, true>(d, tp, ts, i, cand, !move, d.cslot, ref, half, sink);
             else
-                cnt = candidates<CU, false>(d, tp, ts, i, cand, cslot, ref, half, sink);
+                cnt = candidates<CU, false>(d, tp, ts, i, cand, !move, d.cslot, ref, half, sink);
         }
         if (cnt > width) {
             ts.err = ERR_LISTFULL;
