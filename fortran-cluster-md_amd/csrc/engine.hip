@@ -274,6 +274,16 @@ int fill_params(fcmd_system *s, int k, const fcmd_params &p, TrajParams &t)
     t.atskin = p.atskin;
     t.atr = p.sig[0];
     t.sig2f = (float)(p.sig[0] * p.sig[0]);
+    for (int d = 0; d < 3; ++d) {
+        t.boxf[d] = (float)t.box[d];
+        t.iboxf[d] = (float)t.ibox[d];
+    }
+    {
+        // the mixed force kernels' cutoff decision band, formed as they form it
+        const double rc2 = t.atrcut * t.atrcut;
+        t.cut_in2f = (float)(rc2 * (1.0 - 1e-5));
+        t.cut_out2f = (float)(rc2 * (1.0 + 1e-5));
+    }
     t.eps1 = p.eps[0];
     t.dt = p.dt;
     t.Q = p.Q;

@@ -70,6 +70,10 @@ struct TrajParams {
     double e48[MAXS * MAXS], e4[MAXS * MAXS];
     float c1f[MAXS * MAXS], c2f[MAXS * MAXS], c3f[MAXS * MAXS];
     float sig2f;  // sigma^2 of species 1 (the single-species mixed pair sum)
+    // FP32 forms for the mixed pair loops, converted on the host so that the
+    // kernels read them as scalars: box sides, their inverses, and the cutoff
+    // decision band atrcut^2 (1 -+ 1e-5)
+    float boxf[3], iboxf[3], cut_in2f, cut_out2f;
     int natom, nsorts, nbtim, btim, atnenum, ethst_nh;
     int integrator, neighbor, accatom, stoddard;
     int abase, npad, bfirst, nblk;   // atoms, atom blocks
@@ -123,6 +127,9 @@ struct TrajScalars {
     double rs2;                      // list radius^2 of the current list
     double cside[3];                 // cell side used by the current list
     double icside[3];                // 1 / cside
+    // FP32 forms of the current cell grid for the mixed pair loops (scalar
+    // operands there): cells a side, 1/cells to 1 ulp, cell sides
+    float ncf[3], incf[3], csf[3];
     int dmmst, dmmi, step, rebuild, err, npairs, nrebuild, nlist;
     int ncx, ncy, ncz, pad0;
     int nh_iters;

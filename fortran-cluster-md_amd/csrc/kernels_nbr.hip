@@ -502,13 +502,12 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
     int cx, cy, cz;
     unpack_cell(pi.w, cx, cy, cz);
     const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
-    const float ncxf = (float)ncx, ncyf = (float)ncy, nczf = (float)ncz;
-    // 1/n to 1 ulp: dc/n only has to round to the right integer
-    const float icx = __builtin_amdgcn_rcpf(ncxf), icy = __builtin_amdgcn_rcpf(ncyf),
-                icz = __builtin_amdgcn_rcpf(nczf);
-    const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
-    const float bxf = (float)tp.box[0], byf = (float)tp.box[1], bzf = (float)tp.box[2];
-    const float ibxf = (float)tp.ibox[0], ibyf = (float)tp.ibox[1], ibzf = (float)tp.ibox[2];
+    // FP32 grid and box constants as scalars (converted where they are set)
+    const float ncxf = ts.ncf[0], ncyf = ts.ncf[1], nczf = ts.ncf[2];
+    const float icx = ts.incf[0], icy = ts.incf[1], icz = ts.incf[2];
+    const float sx = ts.csf[0], sy = ts.csf[1], sz = ts.csf[2];
+    const float bxf = tp.boxf[0], byf = tp.boxf[1], bzf = tp.boxf[2];
+    const float ibxf = tp.iboxf[0], ibyf = tp.iboxf[1], ibzf = tp.iboxf[2];
     const int lx = ncx >= 3 ? -1 : 0, hx = ncx >= 3 ? 1 : ncx - 1;
     const int ly = ncy >= 3 ? -1 : 0, hy = ncy >= 3 ? 1 : ncy - 1;
     const int lz = ncz >= 3 ? -1 : 0, hz = ncz >= 3 ? 1 : ncz - 1;

@@ -30,6 +30,10 @@ __device__ void set_cell_grid(const TrajParams &tp, TrajScalars &ts)
         n[k] = n[k] < 1 ? 1 : (n[k] > CELL_MAX ? CELL_MAX : n[k]);
         ts.cside[k] = tp.box[k] / (double)n[k];
         ts.icside[k] = 1.0 / ts.cside[k];
+        ts.ncf[k] = (float)n[k];
+        // 1/n to 1 ulp: dc/n only has to round to the right integer
+        ts.incf[k] = __builtin_amdgcn_rcpf(ts.ncf[k]);
+        ts.csf[k] = (float)ts.cside[k];
     }
     if ((long long)n[0] * n[1] * n[2] >= (long long)tp.ncpad) {  // one cell past the grid is kept
         ts.err = ERR_CELLS;
@@ -568,20 +572,21 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
                                                unsigned (*stage)[BLK])
 {
     const float cxi = cell_xf(pi.w), cyi = cell_yf(pi.w), czi = cell_zf(pi.w);
-    const float ncx = (float)ts.ncx, ncy = (float)ts.ncy, ncz = (float)ts.ncz;
-    // 1/n to 1 ulp: dc/n only has to round to the right integer
-    const float icx = __builtin_amdgcn_rcpf(ncx), icy = __builtin_amdgcn_rcpf(ncy),
-                icz = __builtin_amdgcn_rcpf(ncz);
-    const float sx = (float)ts.cside[0], sy = (float)ts.cside[1], sz = (float)ts.cside[2];
-    const float bx = (float)tp.box[0], by = (float)tp.box[1], bz = (float)tp.box[2];
-    const float ibx = (float)tp.ibox[0], iby = (float)tp.ibox[1], ibz = (float)tp.ibox[2];
+    // FP32 grid and box constants, converted where they are set so that they
+    // are scalar operands here (cells a side, 1/cells to 1 ulp, cell sides,
+    // box sides and inverses)
+    const float ncx = ts.ncf[0], ncy = ts.ncf[1], ncz = ts.ncf[2];
+    const float icx = ts.incf[0], icy = ts.incf[1], icz = ts.incf[2];
+    const float sx = ts.csf[0], sy = ts.csf[1], sz = ts.csf[2];
+    const float bx = tp.boxf[0], by = tp.boxf[1], bz = tp.boxf[2];
+    const float ibx = tp.iboxf[0], iby = tp.iboxf[1], ibz = tp.iboxf[2];
     // cutoff decision band: r2 from the cell-offset records is within
-    // ~2e-6 relative of the reference's FP64 r2, so outside +-1e-5 of rc^2 the
-    // FP32 verdict is the reference's; inside, the reference's FP64 test
+    // ~2e-6 relative of the reference's FP64 r2, so outside rc^2 (1 -+ 1e-5)
+    // the FP32 verdict is the reference's; inside, the reference's FP64 test
     // decides (the truncated force jumps at rc, so one pair decided the other
     // way would be the largest error of a large system)
     const double rc2 = tp.atrcut * tp.atrcut;
-    const float in2 = (float)(rc2 * (1.0 - 1e-5)), out2 = (float)(rc2 * (1.0 + 1e-5));
+    const float in2 = tp.cut_in2f, out2 = tp.cut_out2f;
     const float sig2 = tp.sig2f;  // sigma^2 of the single-species form
     const int nn = h.nn;
     double sfx = 0.0, sfy = 0.0, sfz = 0.0;
@@ -700,57 +705,19 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
         pair_sum_fp64<FORM, EN, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, w, stage);
 }
 
-// LJ forces over the full neighbour list plus the integrator's fused
-// epilogue.  MULTI = false: every trajectory of the launch is single-species
-// (ACC_1); MULTI = true: each trajectory uses its own form.  U: neighbours
-// per load batch (a tunable; the pairs are summed in list order whatever U).
-// LT: list entry type.  Single-species kernels of up to 4 neighbours a
-// batch are held to 72 VGPRs (7 wavefronts a SIMD; without spills), which
-// the 16-bit list decode would otherwise just exceed.
-//
-// EN: the energy and virial sums are formed (see launch_force).
-//
-// part: which atom blocks the launch takes.  FP_ALL: every block.
-// FP_KEPT: the blocks of trajectories that keep their list this step (this
-// launch can run beside the rebuild of the others).  FP_REBUILT: the blocks
-// of the trajectories that rebuild, from the rebuild's work list (after it).
-template <typename AccT, bool MULTI, int U, typename LT, bool EN>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U <= 4 ? 7 : 1))) void k_force(
-    DevPtrs d, int mode, int part)
+// The force kernels' epilogue for atom slot i of block b: accelerations from
+// the pair sums (for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:146-152), the
+// integrator's update fused behind them, and the block's partial sums (energy,
+// kinetic terms, virial).  Every thread of the block calls it (act: slot i
+// holds an atom).
+template <typename AccT, bool EN>
+__device__ __forceinline__ void force_finish(const DevPtrs &d, const TrajParams &tp, int b, int i, bool act,
+                                             bool form1, int ti, int mode, double fx, double fy, double fz,
+                                             double e, double w, double (*lds)[4])
 {
     constexpr bool MIXED = sizeof(AccT) == 4;
-    __shared__ double lds[4][4];
-    __shared__ unsigned stage[STAGE][BLK];  // each thread's first list slots
-    int b;
-    if (part == FP_REBUILT) {
-        // one work-list item per workgroup, split over the XCDs in runs
-        const int n = d.wl_cnt[WL_BLK], c = (n + NXCD - 1) / NXCD;
-        const int w = (int)(blockIdx.x % NXCD) * c + (int)(blockIdx.x / NXCD);
-        if ((int)(blockIdx.x / NXCD) >= c || w >= n) return;
-        b = d.wl_blk[w];
-    } else {
-        b = xcd_block(blockIdx.x, d.nblk);
-        if (b >= d.nblk) return;
-    }
-    const int t = d.blk_traj[b];
-    const TrajParams &tp = d.tp[t];
-    const TrajScalars &ts = d.ts[t];
-    if (ts.err) return;
-    if (part == FP_KEPT && ts.rebuild) return;
-    const int i = b * BLK + threadIdx.x;
-    const int rel = i - tp.abase;
-    const bool act = rel < tp.natom;
-    const bool form1 = !MULTI || tp.accatom == ACC_1;
-    double e = 0.0, eks = 0.0, ekfree = 0.0, w = 0.0;
+    double eks = 0.0, ekfree = 0.0;
     if (act) {
-        const int ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
-        // the velocities, loaded before the pair sum so that they arrive
-        // while it runs
-        double fx, fy, fz;
-        if (form1)
-            pair_sum<AccT, ACC_1, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
-        else
-            pair_sum<AccT, ACC_M, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
         // the velocities after the pair sum: loaded before it, they would
         // hold six registers through the pair loop, which the single-species
         // kernel (held to 72) would spill
@@ -760,7 +727,6 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
             vy = d.vy[i];
             vz = d.vz[i];
         }
-        // accelerations: for-a-lj-n-pbc.f:121-127 / for-a-lj-n-pbc-m.f:146-152
         double ax, ay, az;
         double ax0 = 0.0, ay0 = 0.0, az0 = 0.0;
         if (mode == FORCE_ACCATOM) load_acc<AccT>(d, i, ax0, ay0, az0);
@@ -830,6 +796,64 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
             d.part[(size_t)b * NPART + P_SUMV2] = r[1];
         }
     }
+}
+
+// The atom block a force launch's workgroup takes, or -1.  FP_REBUILT: one
+// work-list item per workgroup, split over the XCDs in runs; else the
+// XCD-aware block order.
+__device__ __forceinline__ int force_block(const DevPtrs &d, int part)
+{
+    if (part == FP_REBUILT) {
+        const int n = d.wl_cnt[WL_BLK], c = (n + NXCD - 1) / NXCD;
+        const int w = (int)(blockIdx.x % NXCD) * c + (int)(blockIdx.x / NXCD);
+        if ((int)(blockIdx.x / NXCD) >= c || w >= n) return -1;
+        return d.wl_blk[w];
+    }
+    const int b = xcd_block(blockIdx.x, d.nblk);
+    return b < d.nblk ? b : -1;
+}
+
+// LJ forces over the full neighbour list plus the integrator's fused
+// epilogue.  MULTI = false: every trajectory of the launch is single-species
+// (ACC_1); MULTI = true: each trajectory uses its own form.  U: neighbours
+// per load batch (a tunable; the pairs are summed in list order whatever U).
+// LT: list entry type.  Single-species kernels of up to 4 neighbours a
+// batch are held to 64 VGPRs (8 wavefronts a SIMD, without spills: the
+// FP32 grid and box constants are scalars).
+//
+// EN: the energy and virial sums are formed (see launch_force).
+//
+// part: which atom blocks the launch takes.  FP_ALL: every block.
+// FP_KEPT: the blocks of trajectories that keep their list this step (this
+// launch can run beside the rebuild of the others).  FP_REBUILT: the blocks
+// of the trajectories that rebuild, from the rebuild's work list (after it).
+template <typename AccT, bool MULTI, int U, typename LT, bool EN>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U <= 4 ? 8 : 1))) void k_force(
+    DevPtrs d, int mode, int part)
+{
+    __shared__ double lds[4][4];
+    __shared__ unsigned stage[STAGE][BLK];  // each thread's first list slots
+    const int b = force_block(d, part);
+    if (b < 0) return;
+    const int t = d.blk_traj[b];
+    const TrajParams &tp = d.tp[t];
+    const TrajScalars &ts = d.ts[t];
+    if (ts.err) return;
+    if (part == FP_KEPT && ts.rebuild) return;
+    const int i = b * BLK + threadIdx.x;
+    const int rel = i - tp.abase;
+    const bool act = rel < tp.natom;
+    const bool form1 = !MULTI || tp.accatom == ACC_1;
+    double e = 0.0, w = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    int ti = 0;
+    if (act) {
+        ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
+        if (form1)
+            pair_sum<AccT, ACC_1, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+        else
+            pair_sum<AccT, ACC_M, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+    }
+    force_finish<AccT, EN>(d, tp, b, i, act, form1, ti, mode, fx, fy, fz, e, w, lds);
 }
 
 // Nose-Hoover final velocities v' = (v_h + dth a)/(1 + dth psi'), for

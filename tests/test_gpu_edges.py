@@ -8,6 +8,11 @@ neighbours, and the reference's fatal conditions forced from the host.
   1e13/V (N k T - W/3) (:106,114-117), both on the FP64 and the mixed path,
   on grids of 1 and 5 cells a side and across the periodic face.
 * No neighbours: forces and energy exactly zero.
+* Pairs within 1e-5 of the cutoff on the mixed path: the reference's FP64
+  test on the FP64 positions decides them (for-a-lj-n-pbc.f:89-95), so a
+  rejected pair leaves its atoms exactly no force (its terms are added and
+  taken out again, and the FP64 products of FP32 terms are exact) and an
+  accepted one the oracle's.
 * neighbor-pbc*.f "Skin radius exceeded!!" when the largest displacement
   since the last build exceeds the skin; the Nose-Hoover update reports
   "did not converge" for a thermostat state with 1 + dt/2 psi <= 0, where
@@ -154,3 +159,46 @@ def test_run_ends_with_its_last_steps_energy(precision, nsteps, use_graph):
             sc = s.scalars(t)
             assert sc.step == step
             assert (sc.epot, sc.virial, sc.pressure) == (e, w, p), t
+
+
+def band_case(m=7, seed=9):
+    """m^3 isolated dimers, each at a separation within 5e-6 relative of the
+    cutoff, in random directions, one per cell of an m^3 grid whose spacing
+    keeps every other atom beyond the list radius."""
+    spacing = 2.5 + 2.8 + 0.8  # sigma: cutoff + list radius + margin
+    npairs = m ** 3
+    c = cases.make_case(n=2 * npairs, rho=2 * npairs / (m * spacing) ** 3, seed=seed)
+    L = c["box"][0]
+    sp = L / m
+    rng = np.random.default_rng(seed)
+    g = (np.indices((m, m, m)).reshape(3, -1).T + 0.5) * sp
+    d = rng.normal(size=(npairs, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = c["atrcut"] * (1.0 + rng.uniform(-5e-6, 5e-6, npairs))
+    p1 = g - 0.5 * r[:, None] * d
+    p2 = g + 0.5 * r[:, None] * d
+    pos = np.empty((2 * npairs, 3))
+    pos[0::2], pos[1::2] = p1, p2
+    c["x"], c["y"], c["z"] = pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy()
+    for k in ("vx", "vy", "vz"):
+        c[k] = np.zeros(2 * npairs)
+    return c
+
+
+def test_mixed_cutoff_band_is_decided_in_fp64():
+    import pyoracle as po
+
+    c = band_case()
+    n = c["natom"]
+    o = po.Oracle(c, "mav")
+    o.neighbor(0)
+    o.accatom()
+    ref = np.stack([o.arr[k][:n] for k in ("ax", "ay", "az")], axis=1)
+    taken = np.linalg.norm(ref, axis=1) > 0
+    # both kinds of pair are present
+    assert 0.2 * n < taken.sum() < 0.8 * n
+    got, _ = forces(c, fcmd.MIXED)
+    a = np.stack([got[k] for k in ("ax", "ay", "az")], axis=1)
+    assert np.all(a[~taken] == 0.0)
+    err = np.linalg.norm(a[taken] - ref[taken], axis=1) / np.linalg.norm(ref[taken], axis=1)
+    assert err.max() < 1e-5
