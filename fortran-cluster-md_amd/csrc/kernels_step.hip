@@ -88,26 +88,22 @@ __device__ void neighbor_scalars(const TrajParams &tp, TrajScalars &ts, int *ste
     ts.move = (ts.nrebuild - 1) % tp.resort == 0;
 }
 
-// Fixed-order reduction of one trajectory's per-block partials by one block
-// (deterministic: thread k takes blocks k, k+BLK, ...; then a fixed tree).
-// Returns in thread 0: the sums of slots s0, s1 and s3 (s3 < 0: none) and
-// the max or sum of slot s2.
-__device__ void reduce_partials(const TrajParams &tp, const double *part, int s0, int s1, int s2,
-                                bool max2, int s3, double *lds, double &r0, double &r1, double &r2,
-                                double &r3)
+// Fixed-order reduction of one trajectory's per-block partial slots
+// slot[0..N-1] by one block (deterministic: thread k takes blocks k, k+BLK,
+// ...; then each value's wavefront reduction and (w0 + w1) + (w2 + w3), or
+// the max, with one barrier for all of them).  Results in thread 0.
+template <int N>
+__device__ void reduce_partials(const TrajParams &tp, const double *part, const int (&slot)[N],
+                                const bool (&is_max)[N], double (*lds)[4], double (&r)[N])
 {
-    double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
-    for (int k = threadIdx.x; k < tp.nblk; k += BLK) {
-        const double *p = part + (size_t)(tp.bfirst + k) * NPART;
-        a += p[s0];
-        b += p[s1];
-        c = max2 ? fmax(c, p[s2]) : c + p[s2];
-        if (s3 >= 0) e += p[s3];
+#pragma unroll
+    for (int k = 0; k < N; ++k) r[k] = 0.0;
+    for (int q = threadIdx.x; q < tp.nblk; q += BLK) {
+        const double *p = part + (size_t)(tp.bfirst + q) * NPART;
+#pragma unroll
+        for (int k = 0; k < N; ++k) r[k] = is_max[k] ? fmax(r[k], p[slot[k]]) : r[k] + p[slot[k]];
     }
-    r0 = block_sum(a, lds);
-    r1 = block_sum(b, lds);
-    r2 = max2 ? block_max(c, lds) : block_sum(c, lds);
-    r3 = s3 >= 0 ? block_sum(e, lds) : 0.0;
+    block_reduce_end<N>(r, is_max, lds);
 }
 
 // After the drift: thermostat half update, displacement bookkeeping and the
@@ -261,14 +257,14 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
 // one block per trajectory, after the drift
 __global__ __launch_bounds__(BLK) void k_traj_pre(DevPtrs d)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[2][4];
     const int t = blockIdx.x;
     const TrajParams &tp = d.tp[t];
     TrajScalars &ts = d.ts[t];
     if (ts.err) return;
-    double unused, sumv2, dmmr, none;
-    reduce_partials(tp, d.part, P_EPOT, P_SUMV2, P_DMAX, true, -1, lds, unused, sumv2, dmmr, none);
-    if (threadIdx.x == 0) traj_pre(tp, ts, dmmr, sumv2);
+    double r[2];
+    reduce_partials<2>(tp, d.part, {P_SUMV2, P_DMAX}, {false, true}, lds, r);
+    if (threadIdx.x == 0) traj_pre(tp, ts, r[1], r[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -353,14 +349,14 @@ __device__ void traj_post(const TrajParams &tp, TrajScalars &ts, double esum, do
 // one block per trajectory, after the forces
 __global__ __launch_bounds__(BLK) void k_traj_post(DevPtrs d, int mode, int energy)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[4][4];
     const int t = blockIdx.x;
     const TrajParams &tp = d.tp[t];
     TrajScalars &ts = d.ts[t];
     if (ts.err) return;
-    double esum, eks, ekfree, wsum;
-    reduce_partials(tp, d.part, P_EPOT, P_EKS, P_SUMV2, false, P_VIR, lds, esum, eks, ekfree, wsum);
-    if (threadIdx.x == 0) traj_post(tp, ts, esum, eks, ekfree, wsum, mode, energy != 0);
+    double r[4];
+    reduce_partials<4>(tp, d.part, {P_EPOT, P_EKS, P_SUMV2, P_VIR}, {false, false, false, false}, lds, r);
+    if (threadIdx.x == 0) traj_post(tp, ts, r[0], r[1], r[2], r[3], mode, energy != 0);
 }
 
 // ---------------------------------------------------------------------------
