@@ -625,9 +625,11 @@ __device__ __forceinline__ int candidates(const DevPtrs &d, const TrajParams &tp
 // arrays back.  An atom with more than lwidth
 // neighbours stops the trajectory with LISTFULL (the reference stops when its
 // list array is nearly full, neighbor-pbc.f:89-93); its count is clamped so
-// that no later kernel reads past its slots.
+// that no later kernel reads past its slots.  The 2-candidate form is held to
+// 8 wavefronts a SIMD (its 60 VGPRs allow it; its scalar registers, 106,
+// would hold it to 7): it waits on its loads more than it issues.
 template <int CU, typename LT>
-__global__ __launch_bounds__(BLK) void k_nbr(DevPtrs d)
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(CU == 2 ? 8 : 1))) void k_nbr(DevPtrs d)
 {
     __shared__ int red[2];
     ONE_ITEM(d.wl_blk, WL_BLK, b);
