@@ -488,10 +488,11 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         t.ncblk = (int)(ncpad / SCAN_CHUNK);
         t.lbase = lbase;
         t.lwidth = s->list_width;
+        t.lstride = (int)roundup(s->list_width, LIST_ROUND);
         t.hbase = (int)hbase;
         abase += npad;
         cbase += ncpad;
-        lbase += npad * s->list_width;
+        lbase += npad * t.lstride;
         hbase += p.natom + 1;
     }
     if (abase >= (1LL << 31) || cbase >= (1LL << 31) || lbase >= (1LL << 40)) {
@@ -860,7 +861,7 @@ int fcmd_halflist(fcmd_system *s, int traj, int *atnlist, long long cap, int *at
     std::vector<int> nn(t.npad), oid(t.npad);
     HIPCHK(hipMemcpy(nn.data(), s->d.nn + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(oid.data(), s->d.oid + t.abase, sizeof(int) * t.npad, hipMemcpyDeviceToHost));
-    const long long lo = t.lbase, nslots = (long long)t.npad * t.lwidth;
+    const long long lo = t.lbase, nslots = (long long)t.npad * t.lstride;
     std::vector<int> lst(nslots);
     if (s->g.list16) {
         std::vector<unsigned short> l16(nslots);

@@ -4,8 +4,8 @@
 // system is an ensemble of one).  Trajectory k owns
 //   atoms  [abase, abase + npad)           npad = natom rounded up to BLK
 //   cells  [cbase, cbase + ncpad)          enough for the smallest cell side
-//   list   [lbase, lbase + npad * lwidth)  sliced-ELL neighbour list: per
-//          64-atom tile (one wavefront) lwidth slots x 64 lanes, slot-major
+//   list   [lbase, lbase + npad * lstride) sliced-ELL neighbour list: per
+//          64-atom tile (one wavefront) lstride slots x 64 lanes, slot-major
 // so that every thread block belongs to exactly one trajectory and every
 // per-trajectory reduction is a fixed-order sum over its own blocks.
 //
@@ -78,8 +78,9 @@ struct TrajParams {
     int integrator, neighbor, accatom, stoddard;
     int abase, npad, bfirst, nblk;   // atoms, atom blocks
     int cbase, ncpad, cfirst, ncblk; // cells, cell blocks
-    long long lbase;                 // neighbour-list segment: npad * lwidth slots
+    long long lbase;                 // neighbour-list segment: npad * lstride slots
     int lwidth;                      // list slots per atom (its maximum neighbour count)
+    int lstride;                     // list slots allocated per atom: lwidth rounded up to LIST_ROUND
     int hbase;                       // cluster-size histograms, natom+1 entries
     int nh_mask;                     // species bits coupled to Nose-Hoover
     int nh_refmass;                  // 1: m1 for every coupled atom (reference)
@@ -93,11 +94,17 @@ enum { WL_BLK = 0, WL_CCHK = 1, WL_TRAJ = 2, WL_LISTS = 3 };
 // larger one (a liquid droplet's cells hold 15-30) by a wavefront
 constexpr int CS_REG = 8;
 
+// Every atom owns lstride list slots, lwidth rounded up to a multiple of
+// LIST_ROUND: the force kernel reads its list in batches that may run past
+// the count, and with whole batches allocated no such load needs a bound
+// check (what the slots past the count hold is never used).
+constexpr int LIST_ROUND = 16;
+
 // first list slot of trajectory-relative atom slot rel; its k-th neighbour is
 // at + k*TILE (the 64 lanes of a tile read 256 contiguous bytes per slot)
 __host__ __device__ inline long long list_slot(const TrajParams &tp, int rel)
 {
-    return tp.lbase + (long long)(rel / TILE) * tp.lwidth * TILE + (rel & (TILE - 1));
+    return tp.lbase + (long long)(rel / TILE) * tp.lstride * TILE + (rel & (TILE - 1));
 }
 
 // the list array of entry type LT

@@ -384,7 +384,8 @@ constexpr int SU = 8;
 
 // An atom's list (entries of type LT, trajectory-relative): its count, its
 // first slot and its first SU entries, all loaded at once (every atom owns
-// lwidth slots, whatever its count), before anything waits on memory.
+// lstride >= LIST_ROUND slots, whatever its count), before anything waits on
+// memory.
 // Entries travel as the loaded dwords ("words") and are decoded only once
 // every load of a batch has been issued: a load under a per-lane condition
 // whose result were used at once would make the compiler wait for it inside
@@ -409,7 +410,7 @@ struct ListHead {
         lp = (const unsigned *)(sizeof(LT) == 2 ? p - (rel & 1) : p);
         fill = (unsigned)rel << sh;
 #pragma unroll
-        for (int s = 0; s < SU; ++s) head[s] = s < tp.lwidth ? word(s) : fill;
+        for (int s = 0; s < SU; ++s) head[s] = word(s);
     }
     // the word of list position s, from HBM
     __device__ __forceinline__ unsigned word(int s) const
@@ -417,52 +418,72 @@ struct ListHead {
         constexpr int STRIDE = TILE * (int)sizeof(LT) / 4;  // dwords per list position
         return __builtin_nontemporal_load(lp + (size_t)s * STRIDE);
     }
+    // the trajectory-relative neighbour slot a word holds (below 65536 for
+    // 16-bit entries, so that a gather's byte offset from the trajectory's
+    // first slot is a 32-bit one)
+    __device__ __forceinline__ unsigned decode_rel(unsigned w) const
+    {
+        return sizeof(LT) == 2 ? (w >> sh) & 0xffffu : w;
+    }
     // the neighbour slot a word holds
-    __device__ __forceinline__ int decode(unsigned w) const
-    {
-        return base + (int)(sizeof(LT) == 2 ? (w >> sh) & 0xffffu : w);
-    }
-    // the neighbour slots of positions s0 .. s0+U-1 (the atom itself past
-    // nn); the first STAGE positions come from the LDS stage (all loaded
-    // before the pair loop, so the loop waits only on neighbour gathers)
-    template <int U>
-    __device__ __forceinline__ void batch(const unsigned (*stage)[BLK], int s0, int (&jj)[U]) const
-    {
-        unsigned w[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int s = s0 + u;
-            w[u] = s < nn ? (s < STAGE ? stage[s][threadIdx.x] : word(s)) : fill;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = decode(w[u]);
-    }
+    __device__ __forceinline__ int decode(unsigned w) const { return base + (int)decode_rel(w); }
 };
 
-// Walks an atom's nn list slots in batches of U (U <= SU): load(s0) issues a
-// batch's gathers, sum(s0) consumes them.  With the head slots loaded with
-// the count, the first batch's gathers wait on one round trip instead of
-// two; the other STAGE - SU staged slots are loaded meanwhile and written to
-// LDS after the first batch is in flight.  The stage holds words.
+// Walks an atom's nn list slots in batches of U (U <= SU): load(w) issues the
+// gathers of a batch's words w (the fill word, the atom itself, past nn),
+// sum(s0) consumes them.  Slots below STAGE: the head slots come with the
+// count, so the first batch's gathers wait on one round trip instead of two;
+// the other STAGE - SU are loaded meanwhile and kept in LDS.  Slots from
+// STAGE on (long lists): each batch's words are loaded while the batch
+// before it is summed, so a batch waits on its gathers only.  No word load
+// needs a bound check: an atom owns lstride >= nn slots, a multiple of
+// LIST_ROUND.
 template <int U, typename H, typename Load, typename Sum>
 __device__ __forceinline__ void walk_list(unsigned (*stage)[BLK], const H &h, Load load, Sum sum)
 {
-    static_assert(U <= SU && SU <= STAGE, "batch wider than the early slots");
+    static_assert(U <= SU && SU <= STAGE && STAGE % U == 0 && LIST_ROUND % STAGE == 0,
+                  "batches must tile the staged slots and the allocated ones");
     const int nn = h.nn;
 #pragma unroll
     for (int s = 0; s < SU; ++s) stage[s][threadIdx.x] = s < nn ? h.head[s] : h.fill;
+    // the other staged words, where any lane of the wavefront has that many
+    // neighbours (a wave-uniform branch: the tile row a load reads is fetched
+    // whole anyway, and a row no lane needs is not fetched at all)
     unsigned tail[STAGE - SU];
 #pragma unroll
-    for (int s = SU; s < STAGE; ++s) tail[s - SU] = s < nn ? h.word(s) : h.fill;
+    for (int s = SU; s < STAGE; ++s) {
+        tail[s - SU] = h.fill;
+        if (__builtin_amdgcn_ballot_w64(s < nn)) tail[s - SU] = h.word(s);
+    }
+    unsigned w[U];
     if (nn > 0) {
-        load(0);
 #pragma unroll
-        for (int s = SU; s < STAGE; ++s) stage[s][threadIdx.x] = tail[s - SU];
+        for (int u = 0; u < U; ++u) w[u] = u < nn ? h.head[u] : h.fill;
+        load(w);
+#pragma unroll
+        for (int s = SU; s < STAGE; ++s) stage[s][threadIdx.x] = s < nn ? tail[s - SU] : h.fill;
         sum(0);
     }
-    for (int s0 = U; s0 < nn; s0 += U) {
-        load(s0);
+    for (int s0 = U; s0 < STAGE && s0 < nn; s0 += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = stage[s0 + u][threadIdx.x];
+        load(w);
         sum(s0);
+    }
+    if (nn > STAGE) {
+        unsigned wn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) wn[u] = h.word(STAGE + u);
+        for (int s0 = STAGE; s0 < nn; s0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) w[u] = s0 + u < nn ? wn[u] : h.fill;
+            load(w);
+            if (s0 + U < nn) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) wn[u] = h.word(s0 + U + u);
+            }
+            sum(s0);
+        }
     }
 }
 
@@ -481,8 +502,9 @@ __device__ __forceinline__ void pair_sum_fp64(const DevPtrs &d, const TrajParams
     fx = fy = fz = 0.0;
     int jj[U], tj[U];
     double xj[U], yj[U], zj[U];
-    auto load = [&](int s0) {
-        h.batch(stage, s0, jj);
+    auto load = [&](const unsigned (&wd)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = h.decode(wd[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             xj[u] = d.x[jj[u]];
@@ -642,15 +664,24 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
         sfy = __builtin_fma((double)dyf, fd, sfy);
         sfz = __builtin_fma((double)dzf, fd, sfz);
     };
-    int jj[U], tj[U];
+    int tj[U];
     float4 pj[U];
-    float r2max = 0.0f;  // the largest r2 taken: above in2, a pair in the band was
-    auto load = [&](int s0) {
-        h.batch(stage, s0, jj);
+    // the largest r2 taken, as its bit pattern (r2 >= 0, so integer order is
+    // float order, and the maximum is one integer instruction): above in2, a
+    // pair in the band was
+    int r2max = 0;
+    // the trajectory's records and species: the gathers' offsets are
+    // trajectory-relative
+    const float4 *pt = d.posf + h.base;
+    const int *tt = d.typ + h.base;
+    auto load = [&](const unsigned (&wd)[U]) {
+        unsigned jr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jr[u] = h.decode_rel(wd[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            pj[u] = d.posf[jj[u]];
-            tj[u] = (FORM == ACC_1) ? 0 : d.typ[jj[u]];
+            pj[u] = pt[jr[u]];
+            tj[u] = (FORM == ACC_1) ? 0 : tt[jr[u]];
         }
     };
     auto sum = [&](int s0) {
@@ -661,14 +692,14 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
             const float r2 = separation(pj[u], dxf, dyf, dzf);
             if (r2 <= out2) {
                 pair_terms(dxf, dyf, dzf, r2, tj[u], 1.0f);
-                r2max = fmaxf(r2max, r2);
+                r2max = max(r2max, __float_as_int(r2));
             }
         }
     };
     walk_list<U>(stage, h, load, sum);
     // the rare atom that took a pair in the band walks its list once more
     // and takes out again each such pair the reference's FP64 test rejects
-    if (r2max > in2) {
+    if (r2max > __float_as_int(in2)) {
         for (int q = 0; q < nn; ++q) {
             const int j = h.decode(q < STAGE ? stage[q][threadIdx.x] : h.word(q));
             float dxf, dyf, dzf;

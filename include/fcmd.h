@@ -100,9 +100,11 @@ const char *fcmd_error_text(int code);
  * 2.8-3.3 sigma holds 75-130).  It takes the role of const.inc mxnlist: a list
  * rebuild that finds an atom with more neighbours fails with
  * FCMD_ERR_LISTFULL, as the reference stops when its list array is nearly
- * full (neighbor-pbc.f:89-93).  Memory: 2 * list_width bytes per atom when
- * every trajectory has at most 65536 atom slots (natom rounded up to 256),
- * else 4 * list_width; the environment variable FCMD_LIST_BITS=32, read
+ * full (neighbor-pbc.f:89-93).  Memory: 2 bytes per slot when every
+ * trajectory has at most 65536 atom slots (natom rounded up to 256), else 4,
+ * for list_width rounded up to a multiple of 16 slots per atom (the force
+ * kernel reads whole batches of slots); the environment variable
+ * FCMD_LIST_BITS=32, read
  * here, asks for the 4-byte entries regardless (the results are the same). */
 int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int precision,
                 int list_width);
