@@ -170,9 +170,9 @@ struct fcmd_system {
     DevPtrs d{};
     std::vector<void *> allocs;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;                  // forces beside the rebuild (enqueue_step)
+    hipStream_t side = nullptr;                  // the rebuild beside the forces (enqueue_step)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // forces beside the rebuild: for ensembles (one trajectory either
+    // rebuild beside the forces: for ensembles (one trajectory either
     // rebuilds or not, so it has nothing to overlap); FCMD_OVERLAP=0 or 1
     // in the environment decides instead
     int overlap = 1;
@@ -380,11 +380,14 @@ int capture_graph(fcmd_system *s, int gsteps)
 //
 // The rebuild is a chain of small dependent kernels over the ~1/nbtim of the
 // atoms whose trajectories rebuild now; it leaves most of the GPU idle.  So
-// the forces of the trajectories that keep their lists (which the rebuild
-// does not touch) run on the side stream beside it, and only the rebuilt
-// trajectories' forces follow it.  Every atom block is computed by the same
-// kernel code either way, so the results are the same bits as the forces of
-// all blocks in one launch after the rebuild (s->overlap = 0).
+// it runs on the side stream, followed there by the rebuilt trajectories'
+// forces, while the forces of the trajectories that keep their lists (which
+// the rebuild does not touch) run on the engine stream beside it.  In a step
+// where few trajectories rebuild, the chain takes about as long as those
+// forces, and the cross-stream wait falls on the chain rather than on the
+// forces.  Every atom block is computed by the same kernel code either way,
+// so the results are the same bits as the forces of all blocks in one launch
+// after the rebuild (s->overlap = 0).
 //
 // energy: the step also forms epot, the virial and the pressure.  A run
 // leaves them out of every step whose scalars the next step overwrites
@@ -400,10 +403,10 @@ int enqueue_step(fcmd_system *s, bool energy)
     }
     HIPCHK(hipEventRecord(s->ev_fork, s->stream));
     HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->side, {}, FP_KEPT, energy);
+    launch_rebuild(s->d, s->g, s->mixed, s->side);
+    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->side, {}, FP_REBUILT, energy);
     HIPCHK(hipEventRecord(s->ev_join, s->side));
-    launch_rebuild(s->d, s->g, s->mixed, s->stream);
-    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_REBUILT, energy);
+    launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_KEPT, energy);
     HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
     launch_traj_post(s->d, s->g, FORCE_STEP, s->stream, {}, energy);
     return FCMD_OK;
@@ -541,7 +544,6 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
     A(d.wl_size, (long long)WL_LISTS * ntraj);
     A(d.cell_cnt, cbase) A(d.cell_start, cbase) A(d.cell_first, cbase) A(d.atom_cell, N) A(d.atom_slot, N) A(d.sorted, N);
     A(d.scan_tmp_ll, s->g.ncblk);
-    A(d.big_cnt, 1) A(d.big_cells, N / (CS_REG + 1) + 1);
     A(d.nn, N) A(d.posf, N) A(d.cpos, N) A(d.cslot, N);
     if (s->g.list16) {
         A(d.list16, lbase);

@@ -171,9 +171,6 @@ struct DevPtrs {
     // cells
     int *cell_cnt, *cell_start, *sorted;
     int *cell_first;                 // final trajectory-relative cell starts (cell sort -> list search)
-    int *big_cnt;                    // the cells of more than CS_REG atoms the cell sort left, and
-    int2 *big_cells;                 // each one's (first entry of sorted, atom count); room for
-                                     // natoms / (CS_REG + 1) + 1
     int *atom_cell, *atom_slot;      // per slot at binning: packed cell (CELL_BITS each), slot in cell
     long long *scan_tmp_ll;          // chunk totals for segmented scans
     // neighbour list: nn[slot] entries at list[list_slot(tp, rel) + k*TILE],

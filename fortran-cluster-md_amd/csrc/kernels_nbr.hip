@@ -95,7 +95,6 @@ __global__ __launch_bounds__(BLK) void k_wl_fill(DevPtrs d, int ntraj)
             d.wl_cnt[k] = lds[0][WL_LISTS + k] + lds[1][WL_LISTS + k] + lds[2][WL_LISTS + k] +
                           lds[3][WL_LISTS + k];
     }
-    if (t == 0 && threadIdx.x == 0) d.big_cnt[0] = 0;  // the cell sort's list of large cells
     if (!rebuilding(d.ts[t])) return;
     const TrajParams &tp = d.tp[t];
     for (int q = threadIdx.x; q < tp.nblk; q += BLK) d.wl_blk[off[WL_BLK] + q] = tp.bfirst + q;
@@ -214,7 +213,7 @@ __device__ __forceinline__ double wrap_box(double x, double L)
 // keeps no load ahead of a store it cannot tell apart).
 __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
 {
-    ONE_ITEM(d.wl_blk, WL_BLK, b);
+    FOR_ITEMS(d.wl_blk, WL_BLK, b)
     {
         const int t = d.blk_traj[b];
         const TrajParams &tp = d.tp[t];
@@ -249,7 +248,7 @@ __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
 
 __global__ __launch_bounds__(BLK) void k_scatter(DevPtrs d)
 {
-    ONE_ITEM(d.wl_blk, WL_BLK, b);
+    FOR_ITEMS(d.wl_blk, WL_BLK, b)
     {
         const TrajParams &tp = d.tp[d.blk_traj[b]];
         const int i = b * BLK + threadIdx.x;
@@ -282,8 +281,8 @@ __device__ __forceinline__ void cmp_swap(int &ka, int &va, int &kb, int &vb)
 // runs during the rebuild), and with the final starts in an array of their
 // own no thread waits for another.  A cell of up to CS_REG atoms loads its
 // members and their numbers at once and sorts them in registers (odd-even
-// transposition, CS_REG phases); a larger one goes to a list that k_bigsort
-// sorts a wavefront a cell (a thread sorting 15-30 atoms in place, a chain of
+// transposition, CS_REG phases); a larger one is sorted afterwards by its
+// whole wavefront (a thread sorting 15-30 atoms in place, a chain of
 // dependent loads per step, held the whole launch once droplets had formed).
 constexpr int CS_PARTS = SCAN_CHUNK / BLK;  // workgroups per chunk
 
@@ -317,6 +316,27 @@ __device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
     }
 }
 
+// A large cell (more than CS_REG atoms) sorted by the whole wavefront: lane
+// L holds member L and its reference number, counts the numbers below its
+// own over the cell (one shuffle a member) and stores the member at that
+// rank.  The numbers are distinct, so this is the order the one-thread sort
+// gives.  A cell of more than 64 atoms (never met at liquid density and the
+// list radii here) falls back to lane 0.  Wave-uniform: first and n are the
+// same in every lane.
+__device__ __forceinline__ void sort_cell_wave(const DevPtrs &d, int *a, int n)
+{
+    const int lane = threadIdx.x & 63;
+    if (n > 64) {
+        if (lane == 0) sort_cell(d, a, n);
+        return;
+    }
+    const int v = lane < n ? a[lane] : 0;
+    const int key = lane < n ? d.oid[v] : INT_MAX;
+    int rank = 0;
+    for (int m = 0; m < n; ++m) rank += __shfl(key, m, 64) < key;
+    if (lane < n) a[rank] = v;
+}
+
 __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
 {
     // the work list's chunks, CS_PARTS items each, split over the XCDs as
@@ -336,37 +356,14 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
                            ? d.cell_start[cell + 1]
                            : (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
         d.cell_first[cell] = s0 + add;
-        const int n = e0 - s0;
-        if (n > CS_REG)
-            d.big_cells[atomicAdd(d.big_cnt, 1)] = make_int2(abase + s0 + add, n);
-        else
-            sort_cell(d, d.sorted + abase + s0 + add, n);
-    }
-}
-
-// The large cells, a wavefront each: lane L holds member L and its reference
-// number, counts the numbers below its own over the cell (one shuffle a
-// member) and stores the member at that rank.  The numbers are distinct, so
-// this is the order the one-thread sort gives.  A cell of more than 64 atoms
-// (never met at liquid density and the list radii here) falls back to lane 0.
-__global__ __launch_bounds__(BLK) void k_bigsort(DevPtrs d)
-{
-    const int lane = threadIdx.x & 63;
-    const int nw = (int)(gridDim.x * (BLK / 64));
-    const int n_big = d.big_cnt[0];
-    for (int c = (int)(blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)); c < n_big; c += nw) {
-        const int2 bc = d.big_cells[c];
-        int *a = d.sorted + bc.x;
-        const int n = bc.y;
-        if (n > 64) {
-            if (lane == 0) sort_cell(d, a, n);
-            continue;
+        const int cnt = e0 - s0, first = abase + s0 + add;
+        if (cnt <= CS_REG) sort_cell(d, d.sorted + first, cnt);
+        // the wavefront's large cells, one after another (w is uniform over
+        // the workgroup, so every lane of the wavefront is here)
+        for (unsigned long long big = __builtin_amdgcn_ballot_w64(cnt > CS_REG); big; big &= big - 1) {
+            const int l = __builtin_ctzll(big);
+            sort_cell_wave(d, d.sorted + __shfl(first, l, 64), __shfl(cnt, l, 64));
         }
-        const int v = lane < n ? a[lane] : 0;
-        const int key = lane < n ? d.oid[v] : INT_MAX;
-        int rank = 0;
-        for (int m = 0; m < n; ++m) rank += __shfl(key, m, 64) < key;
-        if (lane < n) a[rank] = v;
     }
 }
 
@@ -393,7 +390,7 @@ __device__ __forceinline__ AccT *acc_array(double *p64, float *p32)
 template <typename AccT>
 __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
 {
-    ONE_ITEM(d.wl_blk, WL_BLK, b);
+    FOR_ITEMS(d.wl_blk, WL_BLK, b)
     {
         const int t = d.blk_traj[b];
         const TrajParams &tp = d.tp[t];
@@ -406,7 +403,7 @@ __global__ __launch_bounds__(BLK) void k_permute(DevPtrs d)
             // cell order, position i of the order holding slot o
             d.cpos[i] = d.posf[o];
             d.cslot[i] = o;
-            return;
+            continue;
         }
         const AccT *ax = acc_array<AccT>(d.ax, d.axf), *ay = acc_array<AccT>(d.ay, d.ayf),
                    *az = acc_array<AccT>(d.az, d.azf);
@@ -714,6 +711,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(CU == 2 ? 8
 }
 
 // ---------------------------------------------------------------------------
+constexpr int REBUILD_LOOP_GRID = 2048;
+
 void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt)
 {
     const KTimer first{kt.t0, nullptr}, last{nullptr, kt.t1};
@@ -722,20 +721,23 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     const dim3 gc(xcd_grid(std::min(g.ncblk, wg)));
     const dim3 gj(xcd_grid(std::min(g.ntraj, wg)));
     launch_timed(k_wl_fill, dim3(g.ntraj), blk, s, first, d, g.ntraj);
-    // one workgroup per possible atom block where a block's work is a short
-    // chain of dependent accesses (no loop: every block starts at once)
+    // The list search: one workgroup per possible atom block (it needs the
+    // registers a loop over blocks would keep).  The binning, the scatter and
+    // the permutation: REBUILD_LOOP_GRID workgroups looping over the rebuilt
+    // blocks (one round of resident wavefronts), which in a step where few
+    // trajectories rebuild spares the dispatch of a workgroup per atom block
+    // of the arena that has nothing to do.
     const dim3 gall(xcd_grid(g.nblk));
-    hipLaunchKernelGGL(k_bin, gall, blk, 0, s, d);
+    const dim3 gl(xcd_grid(std::min(g.nblk, REBUILD_LOOP_GRID)));
+    hipLaunchKernelGGL(k_bin, gl, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
-    hipLaunchKernelGGL(k_scatter, gall, blk, 0, s, d);
+    hipLaunchKernelGGL(k_scatter, gl, blk, 0, s, d);
     hipLaunchKernelGGL(k_cellsort, dim3(xcd_grid(std::min(CS_PARTS * g.ncblk, wg))), blk, 0, s, d);
-    hipLaunchKernelGGL(k_bigsort, dim3(1024), blk, 0, s, d);
     if (mixed)
-        hipLaunchKernelGGL(k_permute<float>, gall, blk, 0, s, d);
+        hipLaunchKernelGGL(k_permute<float>, gl, blk, 0, s, d);
     else
-        hipLaunchKernelGGL(k_permute<double>, gall, blk, 0, s, d);
-    // the search too (it needs the registers a loop would keep)
+        hipLaunchKernelGGL(k_permute<double>, gl, blk, 0, s, d);
     if (g.list16) {
         if (g.nbr_cu == 4)
             launch_timed(k_nbr<4, unsigned short>, gall, blk, s, last, d);
