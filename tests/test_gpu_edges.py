@@ -13,6 +13,8 @@ neighbours, and the reference's fatal conditions forced from the host.
   rejected pair leaves its atoms exactly no force (its terms are added and
   taken out again, and the FP64 products of FP32 terms are exact) and an
   accepted one the oracle's.
+* Crowded cells (27 and 100 atoms in one list cell): the half list equals
+  the oracle's whichever way the cell sort orders a cell's atoms.
 * neighbor-pbc*.f "Skin radius exceeded!!" when the largest displacement
   since the last build exceeds the skin; the Nose-Hoover update reports
   "did not converge" for a thermostat state with 1 + dt/2 psi <= 0, where
@@ -202,3 +204,51 @@ def test_mixed_cutoff_band_is_decided_in_fp64():
     assert np.all(a[~taken] == 0.0)
     err = np.linalg.norm(a[taken] - ref[taken], axis=1) / np.linalg.norm(ref[taken], axis=1)
     assert err.max() < 1e-5
+
+
+def crowded_case():
+    """Vapour around two lattice blobs, each inside one list cell: 27 atoms
+    (3 x 3 x 3 at 0.6 sigma) and 100 atoms (5 x 5 x 4 at 0.5 sigma).  The
+    cell sort gives a cell of more than 8 atoms to its wavefront, and one of
+    more than 64 to the wavefront's first lane."""
+    c = cases.make_case(n=600, rho=0.02, seed=11)
+    L = c["box"][0]
+    side = L / np.floor(L / (c["atrskin"] * (1.0 + 1e-9)))
+    blobs = []
+    for cell, (nx, ny, nz), a in (((2, 3, 4), (3, 3, 3), 0.6), ((7, 6, 5), (5, 5, 4), 0.5)):
+        centre = (np.array(cell) + 0.5) * side
+        g = np.stack(np.meshgrid(*(np.arange(k) - (k - 1) / 2 for k in (nx, ny, nz)), indexing="ij"), -1)
+        blobs.append(centre + a * SIG * g.reshape(-1, 3))
+    blob = np.concatenate(blobs)
+    pos = np.stack([c["x"], c["y"], c["z"]], -1)
+    # vapour atoms clear of the blobs' cells, the blob atoms in their place
+    far = np.ones(len(pos), dtype=bool)
+    for b in blobs:
+        d = pos[:, None, :] - b[None, :, :]
+        d -= L * np.rint(d / L)
+        far &= (np.einsum("ijk,ijk->ij", d, d) > (1.5 * SIG) ** 2).all(axis=1)
+    keep = np.flatnonzero(far)[: 600 - len(blob)]
+    pos = np.concatenate([blob, pos[keep]])
+    assert len(pos) == 600
+    c["x"], c["y"], c["z"] = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    return c
+
+
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+def test_lists_with_crowded_cells(precision):
+    """Cells of 27 and 100 atoms (sorted by reference number a wavefront a
+    cell, and by one lane above 64 atoms): the half list equals the oracle's
+    (neighbor-pbc-mav.f:64-95) after a re-sorting build and after a build
+    that keeps the atoms in their slots."""
+    import pyoracle as po
+
+    c = crowded_case()
+    o = po.Oracle(c, "mav")
+    o.neighbor(0)
+    ref_l, ref_i = o.halflist()
+    s = gpu_system(c, "mav", precision=precision)
+    for _ in range(2):
+        s.neighbor(0)
+        lst, idx = s.halflist()
+        np.testing.assert_array_equal(lst, ref_l)
+        np.testing.assert_array_equal(idx, ref_i)
