@@ -211,6 +211,13 @@ __device__ __forceinline__ double wrap_box(double x, double L)
 // wrap in place (neighbor-pbc.f:62-66) and bin.  Every read comes before the
 // first store, and the slot's atomic before the other stores (the compiler
 // keeps no load ahead of a store it cannot tell apart).
+//
+// A wavefront's atoms lie close to cell order (the slots were sorted by cell
+// at the last re-sort), so neighbouring lanes often bin into the same cell:
+// each run of equal cells in adjacent lanes takes one atomic for the run, its
+// first lane adding the run's length, and the others count on from the
+// result.  The slots within a cell are only distinct positions: the cell
+// sort orders every cell by reference number afterwards.
 __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
 {
     FOR_ITEMS(d.wl_blk, WL_BLK, b)
@@ -219,17 +226,32 @@ __global__ __launch_bounds__(BLK) void k_bin(DevPtrs d)
         const TrajParams &tp = d.tp[t];
         const TrajScalars &ts = d.ts[t];
         const int i = b * BLK + threadIdx.x;
-        if (i - tp.abase < tp.natom) {
-            const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz, cbase = tp.cbase;
+        const bool act = i - tp.abase < tp.natom;  // the active lanes are a prefix of the wavefront
+        const int lane = threadIdx.x & 63;
+        int c = -1, cx = 0, cy = 0, cz = 0;
+        double x = 0.0, y = 0.0, z = 0.0;
+        if (act) {
+            const int ncx = ts.ncx, ncy = ts.ncy, ncz = ts.ncz;
             const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
             const double icx = ts.icside[0], icy = ts.icside[1], icz = ts.icside[2];
             const double x0 = d.x[i], y0 = d.y[i], z0 = d.z[i];  // issued together, before the wraps' branches
-            const double x = wrap_box(x0, bx), y = wrap_box(y0, by), z = wrap_box(z0, bz);
-            const int cx = cell_coord(x, icx, ncx);
-            const int cy = cell_coord(y, icy, ncy);
-            const int cz = cell_coord(z, icz, ncz);
-            const int c = (cz * ncy + cy) * ncx + cx;
-            const int slot = atomicAdd(d.cell_cnt + cbase + c, 1);
+            x = wrap_box(x0, bx), y = wrap_box(y0, by), z = wrap_box(z0, bz);
+            cx = cell_coord(x, icx, ncx);
+            cy = cell_coord(y, icy, ncy);
+            cz = cell_coord(z, icz, ncz);
+            c = (cz * ncy + cy) * ncx + cx;
+        }
+        const int cprev = __shfl_up(c, 1, 64);
+        const bool head = act && (lane == 0 || cprev != c);
+        const unsigned long long heads = __builtin_amdgcn_ballot_w64(head);
+        const int nact = __builtin_popcountll(__builtin_amdgcn_ballot_w64(act));
+        const unsigned long long below = heads & (~0ull >> (63 - lane));  // heads at or below this lane
+        const unsigned long long above = heads & ~(~0ull >> (63 - lane));
+        const int hl = 63 - __builtin_clzll(below | 1ull);                 // this lane's run head
+        int base = 0;
+        if (head) base = atomicAdd(d.cell_cnt + tp.cbase + c, (above ? __builtin_ctzll(above) : nact) - lane);
+        const int slot = __shfl(base, hl, 64) + (lane - hl);
+        if (act) {
             const int packed = cx | (cy << CELL_BITS) | (cz << (2 * CELL_BITS));
             d.x[i] = x;
             d.y[i] = y;
