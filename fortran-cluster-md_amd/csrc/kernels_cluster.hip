@@ -131,21 +131,39 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
     int *ed = d.edges + (size_t)i * EMAX;
     const bool only_a = tp.stoddard == ST_NM;
     if (!(only_a && ti != 0)) {
-        for (int s = 0; s < nn; ++s) {
-            const int j = tp.abase + (int)lp[(size_t)s * TILE];
-            const int tj = d.typ[j];
-            if (only_a && tj != 0) continue;
-            double dx = xi - d.x[j];
-            double dy = yi - d.y[j];
-            double dz = zi - d.z[j];
-            dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
-            dy = dy + by * (bf1 - fint(bf2 + dy * iby));
-            dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= rijc2) {
-                if (deg < EMAX) ed[deg] = j;
-                ++deg;
-                deg1 += (tj == 0);
+        // EB neighbours a batch: the batch's list entries, then their species
+        // and positions, are all loaded before the first edge is stored (the
+        // compiler keeps no load behind a store it cannot tell apart, so a
+        // neighbour at a time was two round trips each); a slot past the
+        // list reads the atom itself
+        constexpr int EB = 4;
+        for (int s0 = 0; s0 < nn; s0 += EB) {
+            int jj[EB], tj[EB];
+            double xj[EB], yj[EB], zj[EB];
+#pragma unroll
+            for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)lp[(size_t)(s0 + u) * TILE] : i;
+#pragma unroll
+            for (int u = 0; u < EB; ++u) {
+                tj[u] = d.typ[jj[u]];
+                xj[u] = d.x[jj[u]];
+                yj[u] = d.y[jj[u]];
+                zj[u] = d.z[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < EB; ++u) {
+                if (s0 + u >= nn || (only_a && tj[u] != 0)) continue;
+                double dx = xi - xj[u];
+                double dy = yi - yj[u];
+                double dz = zi - zj[u];
+                dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
+                dy = dy + by * (bf1 - fint(bf2 + dy * iby));
+                dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= rijc2) {
+                    if (deg < EMAX) ed[deg] = jj[u];
+                    ++deg;
+                    deg1 += (tj[u] == 0);
+                }
             }
         }
     }
@@ -259,10 +277,18 @@ __global__ __launch_bounds__(BLK) void k_cl_rank(DevPtrs d)
 }
 
 // per_atom: also the reference-order per-atom arrays (cs, cek, cgs, clf, ccf)
+// Cluster sizes below HLDS are counted per block in LDS and added to the
+// histograms once per block and size: every root of a size-2 cluster in a
+// trajectory adding to the same two histogram entries had made this kernel
+// wait on one long chain of atomics per entry.
+constexpr int HLDS = 64;
+
 __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d, int per_atom)
 {
     __shared__ double ldsd[4];
     __shared__ int ldsi[4];
+    __shared__ int hcnt[HLDS];
+    __shared__ double hke[HLDS];
     const int b = xcd_block(blockIdx.x, d.nblk);
     if (b >= d.nblk) return;
     const int t = d.blk_traj[b];
@@ -271,6 +297,11 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d, int per_atom)
     const int i = b * BLK + threadIdx.x;
     const bool act = i - tp.abase < tp.natom;
     ClusterScalars &c = d.cls[t];
+    for (int k = threadIdx.x; k < HLDS; k += BLK) {
+        hcnt[k] = 0;
+        hke[k] = 0.0;
+    }
+    __syncthreads();
     // monomers dominate a vapour: their histogram row, the count of clusters
     // and the largest one are per-block partials here, reduced per trajectory
     // in a fixed order by k_cl_sums (no atomics on one address from every block)
@@ -290,6 +321,9 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d, int per_atom)
             if (sz == 1) {
                 mono = 1;
                 mono_ke = d.cek_tmp[i];
+            } else if (sz > 1 && sz < HLDS) {
+                atomicAdd(hcnt + sz, 1);
+                atomicAdd(hke + sz, d.cek_tmp[i]);
             } else if (sz > 1) {
                 atomicAdd(d.o_csi + tp.hbase + sz, 1);
                 atomicAdd(d.o_ceks + tp.hbase + sz, d.cek_tmp[i]);
@@ -302,6 +336,12 @@ __global__ __launch_bounds__(BLK) void k_cl_finish(DevPtrs d, int per_atom)
             d.o_cgs[o] = 0;
         }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < HLDS; k += BLK)
+        if (hcnt[k]) {
+            atomicAdd(d.o_csi + tp.hbase + k, hcnt[k]);
+            atomicAdd(d.o_ceks + tp.hbase + k, hke[k]);
+        }
     const int bm = block_sum_i(mono, ldsi);
     const int bp = block_sum_i(pos, ldsi);
     const double be = block_sum(mono_ke, ldsd);
