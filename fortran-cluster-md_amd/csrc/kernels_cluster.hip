@@ -106,7 +106,15 @@ __global__ __launch_bounds__(BLK) void k_cl_init(DevPtrs d)
 }
 
 // edges within atskin; degrees; liquid-like flag (tWF).  LT: list entry type.
-template <typename LT>
+//
+// REC (the mixed path, whose cell-offset records the drift keeps current):
+// a neighbour's separation comes from its 16-byte record in FP32, as in the
+// force kernel, one gather instead of four.  That r2 is within ~1e-6
+// relative of the reference's FP64 r2, so outside a band of +-1e-4 around
+// atskin^2 its verdict is the reference's; inside it the reference's FP64
+// test on the FP64 positions decides, so the edge set is the reference's
+// bit for bit either way.
+template <typename LT, bool REC>
 __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
 {
     const int b = xcd_block(blockIdx.x, d.nblk);
@@ -119,7 +127,6 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
     const int rel = i - tp.abase;
     if (rel >= tp.natom) return;
     const int ti = d.typ[i];
-    const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
     const double bx = tp.box[0], by = tp.box[1], bz = tp.box[2];
     const double ibx = tp.ibox[0], iby = tp.ibox[1], ibz = tp.ibox[2];
     const double bf1 = fint(ts.atrskin / tp.atr) + 2.0;
@@ -130,39 +137,93 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
     int deg = 0, deg1 = 0;
     int *ed = d.edges + (size_t)i * EMAX;
     const bool only_a = tp.stoddard == ST_NM;
+    const bool species = tp.nsorts > 1;  // one species: every typ is 0, no load
+    // the reference's pair test (stoddard-pbc-n.f:84-89)
+    auto ref_r2 = [&](double xi, double yi, double zi, int j) {
+        double dx = xi - d.x[j];
+        double dy = yi - d.y[j];
+        double dz = zi - d.z[j];
+        dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
+        dy = dy + by * (bf1 - fint(bf2 + dy * iby));
+        dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
+        return dx * dx + dy * dy + dz * dz;
+    };
     if (!(only_a && ti != 0)) {
         // EB neighbours a batch: the batch's list entries, then their species
-        // and positions, are all loaded before the first edge is stored (the
-        // compiler keeps no load behind a store it cannot tell apart, so a
-        // neighbour at a time was two round trips each); a slot past the
-        // list reads the atom itself
+        // and records (or positions), are all loaded before the first edge is
+        // stored (the compiler keeps no load behind a store it cannot tell
+        // apart, so a neighbour at a time was two round trips each); a slot
+        // past the list reads the atom itself
         constexpr int EB = 4;
-        for (int s0 = 0; s0 < nn; s0 += EB) {
-            int jj[EB], tj[EB];
-            double xj[EB], yj[EB], zj[EB];
+        if constexpr (REC) {
+            const float4 pi = d.posf[i];
+            const float cxi = cell_xf(pi.w), cyi = cell_yf(pi.w), czi = cell_zf(pi.w);
+            const float ncx = ts.ncf[0], ncy = ts.ncf[1], ncz = ts.ncf[2];
+            const float icx = ts.incf[0], icy = ts.incf[1], icz = ts.incf[2];
+            const float sx = ts.csf[0], sy = ts.csf[1], sz = ts.csf[2];
+            const float fbx = tp.boxf[0], fby = tp.boxf[1], fbz = tp.boxf[2];
+            const float fibx = tp.iboxf[0], fiby = tp.iboxf[1], fibz = tp.iboxf[2];
+            const float lo2 = (float)(rijc2 * (1.0 - 1e-4)), hi2 = (float)(rijc2 * (1.0 + 1e-4));
+            for (int s0 = 0; s0 < nn; s0 += EB) {
+                int jj[EB], tj[EB];
+                float4 pj[EB];
 #pragma unroll
-            for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)lp[(size_t)(s0 + u) * TILE] : i;
+                for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)lp[(size_t)(s0 + u) * TILE] : i;
 #pragma unroll
-            for (int u = 0; u < EB; ++u) {
-                tj[u] = d.typ[jj[u]];
-                xj[u] = d.x[jj[u]];
-                yj[u] = d.y[jj[u]];
-                zj[u] = d.z[jj[u]];
+                for (int u = 0; u < EB; ++u) {
+                    pj[u] = d.posf[jj[u]];
+                    tj[u] = species ? d.typ[jj[u]] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < EB; ++u) {
+                    if (s0 + u >= nn || (only_a && tj[u] != 0)) continue;
+                    const float4 q = pj[u];
+                    const float cxj = cell_xf(q.w), cyj = cell_yf(q.w), czj = cell_zf(q.w);
+                    const float dxf = ncx < 3.0f ? sep_small(cxi, cxj, sx, pi.x - q.x, fbx, fibx)
+                                                 : sep_wrap(cxi, cxj, ncx, icx, sx, pi.x - q.x);
+                    const float dyf = ncy < 3.0f ? sep_small(cyi, cyj, sy, pi.y - q.y, fby, fiby)
+                                                 : sep_wrap(cyi, cyj, ncy, icy, sy, pi.y - q.y);
+                    const float dzf = ncz < 3.0f ? sep_small(czi, czj, sz, pi.z - q.z, fbz, fibz)
+                                                 : sep_wrap(czi, czj, ncz, icz, sz, pi.z - q.z);
+                    const float r2f = __builtin_fmaf(dxf, dxf, __builtin_fmaf(dyf, dyf, dzf * dzf));
+                    bool in = r2f <= lo2;
+                    if (!in && r2f <= hi2) in = ref_r2(d.x[i], d.y[i], d.z[i], jj[u]) <= rijc2;
+                    if (in) {
+                        if (deg < EMAX) ed[deg] = jj[u];
+                        ++deg;
+                        deg1 += (tj[u] == 0);
+                    }
+                }
             }
+        } else {
+            const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
+            for (int s0 = 0; s0 < nn; s0 += EB) {
+                int jj[EB], tj[EB];
+                double xj[EB], yj[EB], zj[EB];
 #pragma unroll
-            for (int u = 0; u < EB; ++u) {
-                if (s0 + u >= nn || (only_a && tj[u] != 0)) continue;
-                double dx = xi - xj[u];
-                double dy = yi - yj[u];
-                double dz = zi - zj[u];
-                dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
-                dy = dy + by * (bf1 - fint(bf2 + dy * iby));
-                dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= rijc2) {
-                    if (deg < EMAX) ed[deg] = jj[u];
-                    ++deg;
-                    deg1 += (tj[u] == 0);
+                for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)lp[(size_t)(s0 + u) * TILE] : i;
+#pragma unroll
+                for (int u = 0; u < EB; ++u) {
+                    tj[u] = species ? d.typ[jj[u]] : 0;
+                    xj[u] = d.x[jj[u]];
+                    yj[u] = d.y[jj[u]];
+                    zj[u] = d.z[jj[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < EB; ++u) {
+                    if (s0 + u >= nn || (only_a && tj[u] != 0)) continue;
+                    double dx = xi - xj[u];
+                    double dy = yi - yj[u];
+                    double dz = zi - zj[u];
+                    dx = dx + bx * (bf1 - fint(bf2 + dx * ibx));
+                    dy = dy + by * (bf1 - fint(bf2 + dy * iby));
+                    dz = dz + bz * (bf1 - fint(bf2 + dz * ibz));
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 <= rijc2) {
+                        if (deg < EMAX) ed[deg] = jj[u];
+                        ++deg;
+                        deg1 += (tj[u] == 0);
+                    }
                 }
             }
         }
@@ -456,10 +517,20 @@ void launch_size_counts(const DevPtrs &d, const LaunchDims &g, int nmax, int war
 void launch_cluster(const DevPtrs &d, const LaunchDims &g, bool fre, bool per_atom, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cl_init, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
-    if (g.list16)
-        hipLaunchKernelGGL(k_cl_edges<unsigned short>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
-    else
-        hipLaunchKernelGGL(k_cl_edges<int>, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
+    // the mixed path keeps its cell-offset records current (d.cellid: the
+    // cells they are taken against)
+    const dim3 ge(xcd_grid(g.nblk));
+    if (d.cellid) {
+        if (g.list16)
+            hipLaunchKernelGGL((k_cl_edges<unsigned short, true>), ge, dim3(BLK), 0, s, d);
+        else
+            hipLaunchKernelGGL((k_cl_edges<int, true>), ge, dim3(BLK), 0, s, d);
+    } else {
+        if (g.list16)
+            hipLaunchKernelGGL((k_cl_edges<unsigned short, false>), ge, dim3(BLK), 0, s, d);
+        else
+            hipLaunchKernelGGL((k_cl_edges<int, false>), ge, dim3(BLK), 0, s, d);
+    }
     hipLaunchKernelGGL(k_cl_hook, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d);
     hipLaunchKernelGGL(k_cl_count, dim3(xcd_grid(g.nblk)), dim3(BLK), 0, s, d, (int)per_atom);
     if (fre) hipLaunchKernelGGL(k_cl_rank, dim3(g.ntraj), dim3(BLK), 0, s, d);
