@@ -303,39 +303,47 @@ __device__ __forceinline__ void cmp_swap(int &ka, int &va, int &kb, int &vb)
 // runs during the rebuild), and with the final starts in an array of their
 // own no thread waits for another.  A cell of up to CS_REG atoms loads its
 // members and their numbers at once and sorts them in registers (odd-even
-// transposition, CS_REG phases); a larger one is sorted afterwards by its
+// transposition, sort_small); a larger one is sorted afterwards by its
 // whole wavefront (a thread sorting 15-30 atoms in place, a chain of
 // dependent loads per step, held the whole launch once droplets had formed).
 constexpr int CS_PARTS = SCAN_CHUNK / BLK;  // workgroups per chunk
 
-__device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
+// insertion sort of a cell's n atoms by reference number, by one thread
+__device__ __forceinline__ void sort_cell_serial(const DevPtrs &d, int *a, int n)
+{
+    for (int p = 1; p < n; ++p) {
+        const int v = a[p];
+        const int key = d.oid[v];
+        int q = p - 1;
+        while (q >= 0 && d.oid[a[q]] > key) {
+            a[q + 1] = a[q];
+            --q;
+        }
+        a[q + 1] = v;
+    }
+}
+
+// A cell of n <= W atoms sorted in registers by an odd-even transposition
+// network of W phases (the members past n padded with the largest key).  The
+// cell sort picks the narrowest W that holds every cell of its wavefront:
+// in a vapour most cells hold 0-3 atoms, and the 8-wide network had cost
+// every thread 28 compare-exchanges whatever its cell.
+template <int W>
+__device__ __forceinline__ void sort_small(const DevPtrs &d, int *a, int n)
 {
     if (n < 2) return;
-    if (n <= CS_REG) {
-        int v[CS_REG], key[CS_REG];
+    int v[W], key[W];
 #pragma unroll
-        for (int p = 0; p < CS_REG; ++p) v[p] = p < n ? a[p] : 0;
+    for (int p = 0; p < W; ++p) v[p] = p < n ? a[p] : 0;
 #pragma unroll
-        for (int p = 0; p < CS_REG; ++p) key[p] = p < n ? d.oid[v[p]] : INT_MAX;
+    for (int p = 0; p < W; ++p) key[p] = p < n ? d.oid[v[p]] : INT_MAX;
 #pragma unroll
-        for (int ph = 0; ph < CS_REG; ++ph)
+    for (int ph = 0; ph < W; ++ph)
 #pragma unroll
-            for (int p = ph & 1; p + 1 < CS_REG; p += 2) cmp_swap(key[p], v[p], key[p + 1], v[p + 1]);
+        for (int p = ph & 1; p + 1 < W; p += 2) cmp_swap(key[p], v[p], key[p + 1], v[p + 1]);
 #pragma unroll
-        for (int p = 0; p < CS_REG; ++p)
-            if (p < n) a[p] = v[p];
-    } else {
-        for (int p = 1; p < n; ++p) {
-            const int v = a[p];
-            const int key = d.oid[v];
-            int q = p - 1;
-            while (q >= 0 && d.oid[a[q]] > key) {
-                a[q + 1] = a[q];
-                --q;
-            }
-            a[q + 1] = v;
-        }
-    }
+    for (int p = 0; p < W; ++p)
+        if (p < n) a[p] = v[p];
 }
 
 // A large cell (more than CS_REG atoms) sorted by the whole wavefront: lane
@@ -343,13 +351,13 @@ __device__ __forceinline__ void sort_cell(const DevPtrs &d, int *a, int n)
 // own over the cell (one shuffle a member) and stores the member at that
 // rank.  The numbers are distinct, so this is the order the one-thread sort
 // gives.  A cell of more than 64 atoms (never met at liquid density and the
-// list radii here) falls back to lane 0.  Wave-uniform: first and n are the
-// same in every lane.
+// list radii here, but tested) falls back to lane 0's insertion sort.
+// Wave-uniform: a and n are the same in every lane.
 __device__ __forceinline__ void sort_cell_wave(const DevPtrs &d, int *a, int n)
 {
     const int lane = threadIdx.x & 63;
     if (n > 64) {
-        if (lane == 0) sort_cell(d, a, n);
+        if (lane == 0) sort_cell_serial(d, a, n);
         return;
     }
     const int v = lane < n ? a[lane] : 0;
@@ -379,7 +387,13 @@ __global__ __launch_bounds__(BLK) void k_cellsort(DevPtrs d)
                            : (c + 1 < tp.cfirst + tp.ncblk ? (int)d.scan_tmp_ll[c + 1] : tp.natom) - add;
         d.cell_first[cell] = s0 + add;
         const int cnt = e0 - s0, first = abase + s0 + add;
-        if (cnt <= CS_REG) sort_cell(d, d.sorted + first, cnt);
+        const int small = cnt <= CS_REG ? cnt : 0;
+        if (!__builtin_amdgcn_ballot_w64(small > 2))
+            sort_small<2>(d, d.sorted + first, small);
+        else if (!__builtin_amdgcn_ballot_w64(small > 4))
+            sort_small<4>(d, d.sorted + first, small);
+        else
+            sort_small<CS_REG>(d, d.sorted + first, small);
         // the wavefront's large cells, one after another (w is uniform over
         // the workgroup, so every lane of the wavefront is here)
         for (unsigned long long big = __builtin_amdgcn_ballot_w64(cnt > CS_REG); big; big &= big - 1) {
