@@ -36,18 +36,12 @@ struct KTimer {
 
 // a kernel launch that records kt's events, if any
 template <typename F, typename... Args>
-inline void launch_timed_lds(F kern, dim3 grid, dim3 blk, unsigned lds, hipStream_t s, const KTimer &kt,
-                             Args... args)
-{
-    if (kt.t0 || kt.t1)
-        hipExtLaunchKernelGGL(kern, grid, blk, lds, s, kt.t0, kt.t1, 0, args...);
-    else
-        hipLaunchKernelGGL(kern, grid, blk, lds, s, args...);
-}
-template <typename F, typename... Args>
 inline void launch_timed(F kern, dim3 grid, dim3 blk, hipStream_t s, const KTimer &kt, Args... args)
 {
-    launch_timed_lds(kern, grid, blk, 0, s, kt, args...);
+    if (kt.t0 || kt.t1)
+        hipExtLaunchKernelGGL(kern, grid, blk, 0, s, kt.t0, kt.t1, 0, args...);
+    else
+        hipLaunchKernelGGL(kern, grid, blk, 0, s, args...);
 }
 
 // launch_drift = launch_drift_atoms (k_drift) + launch_traj_pre (k_traj_pre);

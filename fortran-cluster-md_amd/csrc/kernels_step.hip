@@ -965,26 +965,17 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
     launch_traj_pre(d, g, s);
 }
 
-// The kept trajectories' force launch runs beside the rebuild chain.  It
-// asks for 4 KB of LDS more than it uses (20.5 KB a workgroup), so that a CU
-// holds 7 of its workgroups instead of 8 (160 KB of LDS a CU) and keeps room
-// for the chain's workgroups, which would otherwise wait for force
-// workgroups to finish.  Measured on the 512-trajectory ensemble: 289 -> 287
-// us a step; 6 or 5 a CU: no gain, slower.
-constexpr unsigned KEPT_LDS = 4096;
-
 template <typename AccT, bool MULTI, typename LT, bool EN>
 static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
                             const KTimer &kt)
 {
     const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
-    const unsigned lds = part == FP_KEPT ? KEPT_LDS : 0;
     if (g.force_u == 2)
-        launch_timed_lds(k_force<AccT, MULTI, 2, LT, EN>, grid, blk, lds, s, kt, d, mode, part);
+        launch_timed(k_force<AccT, MULTI, 2, LT, EN>, grid, blk, s, kt, d, mode, part);
     else if (g.force_u == 8)
-        launch_timed_lds(k_force<AccT, MULTI, 8, LT, EN>, grid, blk, lds, s, kt, d, mode, part);
+        launch_timed(k_force<AccT, MULTI, 8, LT, EN>, grid, blk, s, kt, d, mode, part);
     else
-        launch_timed_lds(k_force<AccT, MULTI, 4, LT, EN>, grid, blk, lds, s, kt, d, mode, part);
+        launch_timed(k_force<AccT, MULTI, 4, LT, EN>, grid, blk, s, kt, d, mode, part);
 }
 
 template <typename AccT, bool MULTI>
