@@ -137,7 +137,6 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
     int deg = 0, deg1 = 0;
     int *ed = d.edges + (size_t)i * EMAX;
     const bool only_a = tp.stoddard == ST_NM;
-    const bool species = tp.nsorts > 1;  // one species: every typ is 0, no load
     // the reference's pair test (stoddard-pbc-n.f:84-89)
     auto ref_r2 = [&](double xi, double yi, double zi, int j) {
         double dx = xi - d.x[j];
@@ -152,9 +151,20 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
         // EB neighbours a batch: the batch's list entries, then their species
         // and records (or positions), are all loaded before the first edge is
         // stored (the compiler keeps no load behind a store it cannot tell
-        // apart, so a neighbour at a time was two round trips each); a slot
-        // past the list reads the atom itself
+        // apart, so a neighbour at a time was two round trips each).  The
+        // entries are loaded whatever the count (an atom owns lstride slots,
+        // a multiple of LIST_ROUND, so a batch never reads past them: a load
+        // under the lane's own condition would be waited for on the spot),
+        // and a slot past the list then names the atom itself.
         constexpr int EB = 4;
+        static_assert(LIST_ROUND % EB == 0, "batches must tile the allocated slots");
+        auto load_batch = [&](int s0, int (&jj)[EB]) {
+            LT w[EB];
+#pragma unroll
+            for (int u = 0; u < EB; ++u) w[u] = lp[(size_t)(s0 + u) * TILE];
+#pragma unroll
+            for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)w[u] : i;
+        };
         if constexpr (REC) {
             const float4 pi = d.posf[i];
             const float cxi = cell_xf(pi.w), cyi = cell_yf(pi.w), czi = cell_zf(pi.w);
@@ -167,12 +177,11 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
             for (int s0 = 0; s0 < nn; s0 += EB) {
                 int jj[EB], tj[EB];
                 float4 pj[EB];
-#pragma unroll
-                for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)lp[(size_t)(s0 + u) * TILE] : i;
+                load_batch(s0, jj);
 #pragma unroll
                 for (int u = 0; u < EB; ++u) {
                     pj[u] = d.posf[jj[u]];
-                    tj[u] = species ? d.typ[jj[u]] : 0;
+                    tj[u] = d.typ[jj[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < EB; ++u) {
@@ -200,11 +209,10 @@ __global__ __launch_bounds__(BLK) void k_cl_edges(DevPtrs d)
             for (int s0 = 0; s0 < nn; s0 += EB) {
                 int jj[EB], tj[EB];
                 double xj[EB], yj[EB], zj[EB];
-#pragma unroll
-                for (int u = 0; u < EB; ++u) jj[u] = s0 + u < nn ? tp.abase + (int)lp[(size_t)(s0 + u) * TILE] : i;
+                load_batch(s0, jj);
 #pragma unroll
                 for (int u = 0; u < EB; ++u) {
-                    tj[u] = species ? d.typ[jj[u]] : 0;
+                    tj[u] = d.typ[jj[u]];
                     xj[u] = d.x[jj[u]];
                     yj[u] = d.y[jj[u]];
                     zj[u] = d.z[jj[u]];
@@ -259,12 +267,28 @@ __global__ __launch_bounds__(BLK) void k_cl_hook(DevPtrs d)
     const int *ed = d.edges + (size_t)i * EMAX;
     const bool fre = tp.stoddard == ST_FRE;
     const bool li = fre && d.liq[i];
-    for (int k = 0; k < deg; ++k) {
-        const int j = ed[k];
-        if (oi < d.oid[j]) {
-            uf_unite(d.parent, d.oid, i, j);
-            if (li && d.liq[j]) uf_unite(d.parentf, d.oid, i, j);
+    // HB edges a batch: their ends and those ends' numbers loaded before the
+    // batch's first union (whose atomics keep later loads behind them); an
+    // atom's EMAX edge slots are whole batches
+    constexpr int HB = 4;
+    static_assert(EMAX % HB == 0, "batches must tile the edge slots");
+    for (int k0 = 0; k0 < deg; k0 += HB) {
+        int jj[HB], oj[HB], lj[HB];
+#pragma unroll
+        for (int u = 0; u < HB; ++u) jj[u] = ed[k0 + u];
+#pragma unroll
+        for (int u = 0; u < HB; ++u) jj[u] = k0 + u < deg ? jj[u] : i;
+#pragma unroll
+        for (int u = 0; u < HB; ++u) {
+            oj[u] = d.oid[jj[u]];
+            lj[u] = li ? d.liq[jj[u]] : 0;
         }
+#pragma unroll
+        for (int u = 0; u < HB; ++u)
+            if (oi < oj[u]) {  // false past the edges (the atom itself)
+                uf_unite(d.parent, d.oid, i, jj[u]);
+                if (lj[u]) uf_unite(d.parentf, d.oid, i, jj[u]);
+            }
     }
 }
 
