@@ -94,9 +94,10 @@ def test_list_entry_width_gives_the_same_bits(precision, monkeypatch):
 @pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_forces_beside_the_rebuild_give_the_same_bits(use_graph, precision, monkeypatch):
-    """The step runs the forces of the trajectories that keep their lists on
-    a side stream beside the rebuild of the others; the same step on one
-    stream (FCMD_OVERLAP=0) must give the same bits, graph or not."""
+    """The step runs the rebuild of the trajectories that rebuild (and then
+    their forces) on a side stream beside the forces of the trajectories
+    that keep their lists; the same step on one stream (FCMD_OVERLAP=0) must
+    give the same bits, graph or not."""
     params, states = _small_ensemble()
     outs = []
     for ov in ("1", "0"):
