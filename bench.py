@@ -17,7 +17,12 @@ interval) is added at that rate, K / cluster_every analyses, so that a short
 window counts what a long run pays per step.
 
 Reported:
-  value            mixed-precision batched path, atom-steps/s
+  value            mixed-precision batched path, atom-steps/s: FP32 pair
+                   arithmetic, each atom's pair forces accumulated in FP32 with
+                   Kahan compensation (FCMD_MIXED_KAHAN), FP64 state,
+                   integrator and reductions
+  mixed_fp64_sums  the same run with each atom's FP32 pair forces summed in
+                   FP64 instead (FCMD_MIXED): throughput, force error
   fp64_batched     the same run in FP64
   fp64_plain       the plain FP64 path: one trajectory at a time (sampled,
                    each on its own captured graph)
@@ -141,7 +146,7 @@ def measure(name, precisions, steps, warmup=10, cluster_every=None):
     every = meta.get("cluster_every", 0) if cluster_every is None else cluster_every
     out = {"atoms": natoms, "trajectories": len(params), "steps": steps}
     for prec in precisions:
-        tag = "mixed" if prec == fcmd.MIXED else "fp64"
+        tag = "fp64" if prec == fcmd.FP64 else "mixed"
         with fcmd.System(params, prec) as s:
             W.setup(s, params, states)
             s.run(warmup, cluster_every=every, use_graph=True)
@@ -155,7 +160,7 @@ def measure(name, precisions, steps, warmup=10, cluster_every=None):
                         "energy_drift_per_tau": float(np.median(np.abs(h1 - h0) / np.abs(h0) / tau)),
                         "temperature_K": float(temperatures(s, params).mean()),
                         "largest_cluster_series": series.max(axis=1).tolist() if series.size else []}
-            if prec == fcmd.MIXED and fcmd.FP64 in precisions:
+            if prec != fcmd.FP64 and fcmd.FP64 in precisions:
                 out["rms_rel_force_err"], _ = force_error(s, params, [0])
         log(f"[bench] {name} {tag}: {out[tag]['atom_steps_per_s']:.4e} atom-steps/s")
     return out
@@ -286,7 +291,7 @@ def main():
     res = {}
     hdrift = {}
     sample = [int(t) for t in np.argsort([p.natom for p in params])[::max(1, len(params) // 16)]]
-    for name, prec in (("mixed", fcmd.MIXED), ("fp64", fcmd.FP64)):
+    for name, prec in (("mixed", fcmd.MIXED_KAHAN), ("mixed_fp64_sums", fcmd.MIXED), ("fp64", fcmd.FP64)):
         log(f"[bench] {name}: setup {len(params)} trajectories, {natoms} atoms")
         s = build_system(params, states, prec)
         # warmup with an analysis: every kernel loaded, the graph captured
@@ -329,6 +334,9 @@ def main():
             for t in range(len(params)):
                 s.download(t)
             res[name]["download_s"] = time.perf_counter() - t0
+        if name == "mixed_fp64_sums":
+            res[name]["ferr"] = force_error(s, params, sample)
+            log(f"[bench] mixed_fp64_sums: RMS relative force error {res[name]['ferr'][0]:.3e}")
         s.close()
 
     plain = None
@@ -371,9 +379,9 @@ def main():
     extra = None
     if not args.no_extra:
         extra = {
-            "lj_nh_262k": measure("lj_nh_262k", (fcmd.FP64, fcmd.MIXED), steps=1000),
-            "lj_nh_2m": measure("lj_nh_2m", (fcmd.FP64, fcmd.MIXED), steps=400, cluster_every=50),
-            "mixture64": measure("mixture64", (fcmd.MIXED,), steps=400),
+            "lj_nh_262k": measure("lj_nh_262k", (fcmd.FP64, fcmd.MIXED_KAHAN), steps=1000),
+            "lj_nh_2m": measure("lj_nh_2m", (fcmd.FP64, fcmd.MIXED_KAHAN), steps=400, cluster_every=50),
+            "mixture64": measure("mixture64", (fcmd.MIXED_KAHAN,), steps=400),
         }
 
     cpu = None
@@ -411,6 +419,13 @@ def main():
         "fp64_plain": plain,
         "speedup_vs_fp64_plain": value / plain if plain else None,
         "speedup_vs_fp64_batched": value / res["fp64"]["rate"],
+        "mixed_fp64_sums": {"value": res["mixed_fp64_sums"]["rate"],
+                            "ms_per_step": res["mixed_fp64_sums"]["wall"] / K * 1e3,
+                            "rms_rel_force_err": res["mixed_fp64_sums"]["ferr"][0],
+                            "rms_rel_force_err_worst_trajectory": res["mixed_fp64_sums"]["ferr"][1],
+                            "energy_drift_per_tau": float(np.median(hdrift["mixed_fp64_sums"])),
+                            "temperature_K": float(res["mixed_fp64_sums"]["temp"].mean()),
+                            "largest_cluster_series": res["mixed_fp64_sums"]["series"]},
         "rms_rel_force_err": ferr,
         "rms_rel_force_err_worst_trajectory": ferr_worst,
         "force_err_sample": f"{len(sample)} trajectories of the batch, on the mixed run's state",

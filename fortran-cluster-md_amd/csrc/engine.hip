@@ -448,6 +448,7 @@ const char *fcmd_error_text(int code)
 int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int precision, int list_width)
 {
     if (!out || ntraj <= 0 || !params) return FCMD_ERR_ARG;
+    if (precision != FCMD_FP64 && precision != FCMD_MIXED && precision != FCMD_MIXED_KAHAN) return FCMD_ERR_ARG;
     *out = nullptr;
     if (fcmd_device_count() <= 0) return FCMD_ERR_NODEVICE;
     fcmd_system *s = new (std::nothrow) fcmd_system();
@@ -458,7 +459,8 @@ int fcmd_create(fcmd_system **out, int ntraj, const fcmd_params *params, int pre
         return rc0;
     }
     s->ntraj = ntraj;
-    s->mixed = precision == FCMD_MIXED;
+    s->mixed = precision == FCMD_MIXED || precision == FCMD_MIXED_KAHAN;
+    s->g.kahan = precision == FCMD_MIXED_KAHAN;
     if (const char *rs = std::getenv("FCMD_RESORT")) s->resort = std::max(1, std::atoi(rs));
     s->list_width = list_width > 0 ? list_width : 256;
     s->hp.assign(params, params + ntraj);

@@ -21,10 +21,11 @@ struct LaunchDims {
     int nhblk;   // histogram chunks
     int ntraj;
     int list16;  // list entries are 16-bit (DevPtrs::list16)
+    int kahan;   // mixed path: FP32 per-atom force sums with Kahan compensation (FCMD_MIXED_KAHAN)
     // tunables (none changes a result bit; fcmd_autotune picks them):
     int force_u;  // neighbours per load batch in the force kernel: 2, 4 or 8
     int nbr_cu;   // candidates per load batch in the list search: 2 or 4
-    int wl_grid;  // workgroups of each rebuild kernel: 1024, 2048 or 4096
+    int wl_grid;  // workgroups of the rebuild's cell-count scans: 1024, 2048 or 4096
 };
 
 // Events recorded at a kernel's start and end (hipExtLaunchKernelGGL), for

@@ -583,7 +583,19 @@ __device__ __forceinline__ bool reference_cut_test(const DevPtrs &d, const TrajP
     return dx * dx + dy * dy + dz * dz <= rc2;
 }
 
-template <int FORM, bool EN, bool SMALL, int U, typename H>
+// Kahan-compensated FP32 addition: s + x with the rounding error of earlier
+// additions carried in c (Kahan 1965)
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x)
+{
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// KAHAN: each pair's FP32 force components are accumulated per atom in FP32
+// with Kahan compensation (FCMD_MIXED_KAHAN) instead of in FP64.
+template <int FORM, bool EN, bool SMALL, bool KAHAN, int U, typename H>
 __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                                int i, const float4 &pi, const H &h, int ti, double &fx,
                                                double &fy, double &fz, double &e, double &w,
@@ -608,6 +620,7 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
     const float sig2 = tp.sig2f;  // sigma^2 of the single-species form
     const int nn = h.nn;
     double sfx = 0.0, sfy = 0.0, sfz = 0.0;
+    float kfx = 0.0f, kfy = 0.0f, kfz = 0.0f, kcx = 0.0f, kcy = 0.0f, kcz = 0.0f;
     // this atom's energy and virial terms, summed in FP32 (its own few
     // pairs) and reduced over atoms in FP64 by the caller
     float ef = 0.0f, wf = 0.0f;
@@ -659,10 +672,17 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
                 wf = __builtin_fmaf(sgn * fij, r2, wf);
             }
         }
-        const double fd = (double)(sgn * fij);
-        sfx = __builtin_fma((double)dxf, fd, sfx);
-        sfy = __builtin_fma((double)dyf, fd, sfy);
-        sfz = __builtin_fma((double)dzf, fd, sfz);
+        if constexpr (KAHAN) {
+            const float fs = sgn * fij;
+            kahan_add(kfx, kcx, dxf * fs);
+            kahan_add(kfy, kcy, dyf * fs);
+            kahan_add(kfz, kcz, dzf * fs);
+        } else {
+            const double fd = (double)(sgn * fij);
+            sfx = __builtin_fma((double)dxf, fd, sfx);
+            sfy = __builtin_fma((double)dyf, fd, sfy);
+            sfz = __builtin_fma((double)dzf, fd, sfz);
+        }
     };
     int tj[U];
     float4 pj[U];
@@ -708,14 +728,20 @@ __device__ __forceinline__ void pair_sum_mixed(const DevPtrs &d, const TrajParam
                 pair_terms(dxf, dyf, dzf, r2, (FORM == ACC_1) ? 0 : d.typ[j], -1.0f);
         }
     }
-    fx = sfx;
-    fy = sfy;
-    fz = sfz;
+    if constexpr (KAHAN) {
+        fx = kfx;
+        fy = kfy;
+        fz = kfz;
+    } else {
+        fx = sfx;
+        fy = sfy;
+        fz = sfz;
+    }
     e += (double)ef;
     w += (double)wf;
 }
 
-template <typename AccT, int FORM, bool EN, int U, typename LT>
+template <typename AccT, int FORM, bool EN, bool KAHAN, int U, typename LT>
 __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp, const TrajScalars &ts,
                                          int i, int rel, int ti, double &fx, double &fy, double &fz,
                                          double &e, double &w, unsigned (*stage)[BLK])
@@ -725,9 +751,9 @@ __device__ __forceinline__ void pair_sum(const DevPtrs &d, const TrajParams &tp,
     if constexpr (sizeof(AccT) == 4) {
         const float4 pi = d.posf[i];
         if (ts.ncx < 3 || ts.ncy < 3 || ts.ncz < 3)
-            pair_sum_mixed<FORM, EN, true, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
+            pair_sum_mixed<FORM, EN, true, KAHAN, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
         else
-            pair_sum_mixed<FORM, EN, false, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
+            pair_sum_mixed<FORM, EN, false, KAHAN, U>(d, tp, ts, i, pi, h, ti, fx, fy, fz, e, w, stage);
     } else
         pair_sum_fp64<FORM, EN, U>(d, tp, ts, i, h, ti, fx, fy, fz, e, w, stage);
 }
@@ -854,7 +880,7 @@ __device__ __forceinline__ int force_block(const DevPtrs &d, int part)
 // FP_KEPT: the blocks of trajectories that keep their list this step (this
 // launch can run beside the rebuild of the others).  FP_REBUILT: the blocks
 // of the trajectories that rebuild, from the rebuild's work list (after it).
-template <typename AccT, bool MULTI, int U, typename LT, bool EN>
+template <typename AccT, bool MULTI, int U, typename LT, bool EN, bool KAHAN>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U <= 4 ? 8 : 1))) void k_force(
     DevPtrs d, int mode, int part)
 {
@@ -876,9 +902,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(!MULTI && U
     if (act) {
         ti = tp.nsorts > 1 ? d.typ[i] : 0;  // one species: no load
         if (form1)
-            pair_sum<AccT, ACC_1, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+            pair_sum<AccT, ACC_1, EN, KAHAN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
         else
-            pair_sum<AccT, ACC_M, EN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
+            pair_sum<AccT, ACC_M, EN, KAHAN, U, LT>(d, tp, ts, i, rel, ti, fx, fy, fz, e, w, stage);
     }
     force_finish<AccT, EN>(d, tp, b, i, act, form1, ti, mode, fx, fy, fz, e, w, lds);
 }
@@ -965,17 +991,28 @@ void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t
     launch_traj_pre(d, g, s);
 }
 
+template <typename AccT, bool MULTI, typename LT, bool EN, bool KAHAN>
+static void launch_force_k(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
+                           const KTimer &kt)
+{
+    const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
+    if (g.force_u == 2)
+        launch_timed(k_force<AccT, MULTI, 2, LT, EN, KAHAN>, grid, blk, s, kt, d, mode, part);
+    else if (g.force_u == 8)
+        launch_timed(k_force<AccT, MULTI, 8, LT, EN, KAHAN>, grid, blk, s, kt, d, mode, part);
+    else
+        launch_timed(k_force<AccT, MULTI, 4, LT, EN, KAHAN>, grid, blk, s, kt, d, mode, part);
+}
+
+// the Kahan-compensated FP32 sums exist on the mixed path only
 template <typename AccT, bool MULTI, typename LT, bool EN>
 static void launch_force_lt(const DevPtrs &d, const LaunchDims &g, int mode, int part, hipStream_t s,
                             const KTimer &kt)
 {
-    const dim3 grid(xcd_grid(g.nblk)), blk(BLK);
-    if (g.force_u == 2)
-        launch_timed(k_force<AccT, MULTI, 2, LT, EN>, grid, blk, s, kt, d, mode, part);
-    else if (g.force_u == 8)
-        launch_timed(k_force<AccT, MULTI, 8, LT, EN>, grid, blk, s, kt, d, mode, part);
+    if (sizeof(AccT) == 4 && g.kahan)
+        launch_force_k<AccT, MULTI, LT, EN, sizeof(AccT) == 4>(d, g, mode, part, s, kt);
     else
-        launch_timed(k_force<AccT, MULTI, 4, LT, EN>, grid, blk, s, kt, d, mode, part);
+        launch_force_k<AccT, MULTI, LT, EN, false>(d, g, mode, part, s, kt);
 }
 
 template <typename AccT, bool MULTI>

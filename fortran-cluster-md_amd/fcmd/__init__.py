@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libfcmd.so")
 CIKB = 1.38066e-6  # const.inc:17, Boltzmann constant in internal units
 CIMU = 1.66056     # const.inc:18
 
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_KAHAN = 0, 1, 2
 INT_VV, INT_MAV, INT_NH = 0, 1, 2
 NB_PBC, NB_MAV = 0, 1
 ACC_1, ACC_M, ACC_TABLE = 0, 1, 2

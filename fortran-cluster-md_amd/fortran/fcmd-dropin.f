@@ -85,6 +85,7 @@
 	env = ' '
 	call get_environment_variable('FCMD_PRECISION', env)
 	if (env(1:5).eq.'mixed') prec = FCMD_MIXED
+	if (env(1:5).eq.'kahan') prec = FCMD_MIXED_KAHAN
 	env = ' '
 	call get_environment_variable('FCMD_RESIDENT', env)
 	fcmd_resident = env(1:1).ne.'0'

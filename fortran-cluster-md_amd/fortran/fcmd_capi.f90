@@ -6,7 +6,7 @@ module fcmd_capi
   implicit none
 
   integer(c_int), parameter :: FCMD_MAXSORTS = 10
-  integer(c_int), parameter :: FCMD_FP64 = 0, FCMD_MIXED = 1
+  integer(c_int), parameter :: FCMD_FP64 = 0, FCMD_MIXED = 1, FCMD_MIXED_KAHAN = 2
   integer(c_int), parameter :: FCMD_INT_VV = 0, FCMD_INT_MAV = 1, FCMD_INT_NH = 2
   integer(c_int), parameter :: FCMD_NB_PBC = 0, FCMD_NB_MAV = 1
   integer(c_int), parameter :: FCMD_ACC_1 = 0, FCMD_ACC_M = 1, FCMD_ACC_TABLE = 2

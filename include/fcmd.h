@@ -32,7 +32,7 @@ extern "C" {
 
 #define FCMD_MAXSORTS 10
 
-enum { FCMD_FP64 = 0, FCMD_MIXED = 1 };
+enum { FCMD_FP64 = 0, FCMD_MIXED = 1, FCMD_MIXED_KAHAN = 2 };
 enum { FCMD_INT_VV = 0, FCMD_INT_MAV = 1, FCMD_INT_NH = 2 };
 enum { FCMD_NB_PBC = 0, FCMD_NB_MAV = 1 };
 enum { FCMD_ACC_1 = 0, FCMD_ACC_M = 1, FCMD_ACC_TABLE = 2 };
@@ -94,7 +94,11 @@ int fcmd_device_count(void);
 const char *fcmd_error_text(int code);
 
 /* Allocate an ensemble of ntraj trajectories on the current HIP device.
- * precision: FCMD_FP64 or FCMD_MIXED.  list_width: neighbour-list slots per
+ * precision: FCMD_FP64 (the reference's arithmetic); FCMD_MIXED (pair
+ * separations and LJ terms in FP32, each atom's pair forces summed in FP64;
+ * state, integrator and reductions over atoms in FP64); FCMD_MIXED_KAHAN
+ * (the same, each atom's pair forces summed in FP32 with Kahan
+ * compensation); any other value is FCMD_ERR_ARG.  list_width: neighbour-list slots per
  * atom, i.e. the most neighbours (within atrskin, both directions) any atom
  * may have; 0 selects 256 (a liquid droplet at the adaptive skin's usual
  * 2.8-3.3 sigma holds 75-130).  It takes the role of const.inc mxnlist: a list

@@ -57,6 +57,17 @@ def test_create_without_device_fails_loudly():
         fcmd.System(p)
 
 
+def test_create_rejects_an_unknown_precision():
+    """fcmd_create checks its precision argument before anything else is
+    set up (no device needed)."""
+    import ctypes as C
+    p = fcmd.Params()
+    p.natom = 10
+    h = C.c_void_p()
+    assert fcmd.lib().fcmd_create(C.byref(h), 1, C.byref(p), 3, 0) == 10  # FCMD_ERR_ARG
+    assert not h.value
+
+
 def test_init_random_contract():
     box = np.array([120.0, 120.0, 120.0])
     n = 2000

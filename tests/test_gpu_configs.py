@@ -151,13 +151,15 @@ def test_ab_dimer_at_eps_ab_03(precision):
         assert abs(sc.virial - f * r) <= tol * fscale * SIG
 
 
-def test_ensemble_mixed_precision_against_the_oracle():
+@pytest.mark.parametrize("precision", [fcmd.MIXED, fcmd.MIXED_KAHAN])
+def test_ensemble_mixed_precision_against_the_oracle(precision):
     """64 trajectories drawn like ensemble512's, advanced in mixed precision
-    with graph replay and analyses; on each trajectory's snapshot the forces
-    are within 1e-5 RMS of the oracle's FP64 AccAtom on the same list, energy
-    and virial within 1e-5, and the Stillinger labels bit-exact."""
+    (pair forces summed in FP64, or in FP32 with Kahan compensation) with
+    graph replay and analyses; on each trajectory's snapshot the forces are
+    within 1e-5 RMS of the oracle's FP64 AccAtom on the same list, energy and
+    virial within 1e-5, and the Stillinger labels bit-exact."""
     params, states, meta = W.ensemble512(ntraj=64)
-    with fcmd.System(params, fcmd.MIXED) as s:
+    with fcmd.System(params, precision) as s:
         W.setup(s, params, states)
         series, fpt = s.run(100, cluster_every=20, fpt_size=50, use_graph=True)
         assert series.shape == (5, 64)

@@ -64,7 +64,7 @@ def forces(case, precision, with_virial=False):
     return s.download(), sc.epot
 
 
-@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED, fcmd.MIXED_KAHAN])
 @pytest.mark.parametrize("box_sig", [5.0, 15.0])
 @pytest.mark.parametrize("across", [False, True])
 def test_lj_dimer_closed_forms(precision, box_sig, across):
@@ -187,7 +187,8 @@ def band_case(m=7, seed=9):
     return c
 
 
-def test_mixed_cutoff_band_is_decided_in_fp64():
+@pytest.mark.parametrize("precision", [fcmd.MIXED, fcmd.MIXED_KAHAN])
+def test_mixed_cutoff_band_is_decided_in_fp64(precision):
     import pyoracle as po
 
     c = band_case()
@@ -199,7 +200,7 @@ def test_mixed_cutoff_band_is_decided_in_fp64():
     taken = np.linalg.norm(ref, axis=1) > 0
     # both kinds of pair are present
     assert 0.2 * n < taken.sum() < 0.8 * n
-    got, _ = forces(c, fcmd.MIXED)
+    got, _ = forces(c, precision)
     a = np.stack([got[k] for k in ("ax", "ay", "az")], axis=1)
     assert np.all(a[~taken] == 0.0)
     err = np.linalg.norm(a[taken] - ref[taken], axis=1) / np.linalg.norm(ref[taken], axis=1)

@@ -256,12 +256,16 @@ def test_ensemble_matches_single_runs():
         assert ens.scalars(t).epot == s.scalars().epot
 
 
-def test_mixed_precision_force_error():
+@pytest.mark.parametrize("precision", [fcmd.MIXED, fcmd.MIXED_KAHAN])
+def test_mixed_precision_force_error(precision):
+    """Both mixed forms (FP32 pair terms summed in FP64, or in FP32 with
+    Kahan compensation): RMS relative force error, energy and virial within
+    1e-5 of the oracle's FP64 AccAtom on a system with liquid-like clusters."""
     case = cases.make_case(n=4000, rho=0.05, seed=41, dense_frac=0.4, ncenters=5)
     o = po.Oracle(case, "mav")
     o.neighbor(0)
     o.accatom()
-    s = gpu_system(case, "mav", precision=fcmd.MIXED)
+    s = gpu_system(case, "mav", precision=precision)
     s.neighbor(0)
     s.upload({"ax": np.zeros(4000), "ay": np.zeros(4000), "az": np.zeros(4000)})
     s.accatom()
@@ -274,15 +278,16 @@ def test_mixed_precision_force_error():
     assert abs(sc.virial - o.st.virial) < 1e-5 * abs(o.st.virial)
 
 
+@pytest.mark.parametrize("precision", [fcmd.MIXED, fcmd.MIXED_KAHAN])
 @pytest.mark.parametrize("n,rho", [(8, 0.05), (12, 0.05), (100, 0.05)])
-def test_mixed_precision_small_boxes(n, rho):
+def test_mixed_precision_small_boxes(n, rho, precision):
     """Boxes of 1, 2 and 4 cells a side (list radius 2.8 sigma): the mixed
     path's cell-offset separations need the rounding image below 3 cells."""
     case = cases.make_case(n=n, rho=rho, seed=43, dense_frac=0.0)
     o = po.Oracle(case, "mav")
     o.neighbor(0)
     o.accatom()
-    s = gpu_system(case, "mav", precision=fcmd.MIXED)
+    s = gpu_system(case, "mav", precision=precision)
     s.neighbor(0)
     s.upload({"ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n)})
     s.accatom()
