@@ -55,10 +55,11 @@ def _h_b(s, p, t):
     return sc.ekin + sc.epot + sc.psi ** 2 * p.Q / 2 + 3 * p.num[1] * fcmd.CIKB * p.att * sc.s
 
 
-def test_mixture64_full_size_mixed():
+@pytest.mark.parametrize("precision", [fcmd.MIXED, fcmd.MIXED_KAHAN])
+def test_mixture64_full_size_mixed(precision):
     params, states, meta = W.mixture64()
     assert len(params) == 64 and all(p.natom == 65536 for p in params)
-    with fcmd.System(params, fcmd.MIXED) as s:
+    with fcmd.System(params, precision) as s:
         W.setup(s, params, states)
         h0 = np.array([_h_b(s, p, t) for t, p in enumerate(params)])
         series, fpt = s.run(200, cluster_every=100, use_graph=True)
@@ -114,7 +115,7 @@ def test_mixture_table_with_lb_values_matches_the_reference_form():
                 assert abs(sc.epot - o.st.epot) < 10 * tol * abs(o.st.epot)
 
 
-@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED])
+@pytest.mark.parametrize("precision", [fcmd.FP64, fcmd.MIXED, fcmd.MIXED_KAHAN])
 def test_ab_dimer_at_eps_ab_03(precision):
     """V(r) = 4 eps_AB ((s/r)^12 - (s/r)^6) and F(r) = 48 eps_AB/r
     ((s/r)^12 - (s/r)^6/2) for an A-B pair with eps_AB = 0.3 eps (not the

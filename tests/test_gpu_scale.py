@@ -69,7 +69,8 @@ def _sample_completeness(case, lst, idx, nsample, rng):
 
 
 @pytest.mark.parametrize("workload,prec", [("lj_nh_262k", fcmd.FP64), ("lj_nh_262k", fcmd.MIXED),
-                                           ("lj_nh_2m", fcmd.FP64), ("lj_nh_2m", fcmd.MIXED)])
+                                           ("lj_nh_262k", fcmd.MIXED_KAHAN), ("lj_nh_2m", fcmd.FP64),
+                                           ("lj_nh_2m", fcmd.MIXED), ("lj_nh_2m", fcmd.MIXED_KAHAN)])
 def test_full_size_forces_lists_clusters(workload, prec):
     params, states, meta = W.WORKLOADS[workload]()
     p, st = params[0], states[0]
