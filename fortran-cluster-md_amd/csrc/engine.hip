@@ -397,13 +397,13 @@ int enqueue_step(fcmd_system *s, bool energy)
 {
     launch_drift(s->d, s->g, s->mixed, s->stream);
     if (!s->overlap) {
-        launch_rebuild(s->d, s->g, s->mixed, s->stream);
+        launch_rebuild(s->d, s->g, s->mixed, s->stream, {}, true);
         launch_force(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, energy);
         return FCMD_OK;
     }
     HIPCHK(hipEventRecord(s->ev_fork, s->stream));
     HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-    launch_rebuild(s->d, s->g, s->mixed, s->side);
+    launch_rebuild(s->d, s->g, s->mixed, s->side, {}, true);
     launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->side, {}, FP_REBUILT, energy);
     HIPCHK(hipEventRecord(s->ev_join, s->side));
     launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, {}, FP_KEPT, energy);
@@ -1115,7 +1115,7 @@ int fcmd_autotune(fcmd_system *s, int rounds, int *tuning)
         g.wl_grid = WL_GRIDS[(k % nr) % 3];
         launch_drift(s->d, g, s->mixed, s->stream);
         HIPCHK(hipEventRecord(e[0], s->stream));
-        launch_rebuild(s->d, g, s->mixed, s->stream);
+        launch_rebuild(s->d, g, s->mixed, s->stream, {}, true);
         HIPCHK(hipEventRecord(e[1], s->stream));
         // the run's kernels: the energy on the last step only
         const bool en = k == nsteps - 1;
@@ -1167,7 +1167,7 @@ int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms)
         const KTimer kt[5] = {{e[0], e[1]}, {e[2], e[3]}, {e[4], e[5]}, {e[6], e[7]}, {e[8], e[9]}};
         launch_drift_atoms(s->d, s->g, s->mixed, s->stream, kt[0]);
         launch_traj_pre(s->d, s->g, s->stream, kt[1]);
-        launch_rebuild(s->d, s->g, s->mixed, s->stream, kt[2]);
+        launch_rebuild(s->d, s->g, s->mixed, s->stream, kt[2], true);
         // as a run's captured graph does: the energy every 10th step
         const bool en = (k + 1) % 10 == 0 || k == nsteps - 1;
         launch_force_atoms(s->d, s->g, s->mixed, s->single_species, FORCE_STEP, s->stream, kt[3], FP_ALL, en);

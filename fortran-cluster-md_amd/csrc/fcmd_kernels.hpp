@@ -62,7 +62,10 @@ void launch_traj_post(const DevPtrs &d, const LaunchDims &g, int mode, hipStream
                       bool energy = true);
 // applies any pending Nose-Hoover final update and clears the flags
 void launch_kick_nh(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s);
-void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt = {});
+// after_pre: the step's k_traj_pre ran just before (for a single trajectory
+// it wrote the work lists, so k_wl_fill is left out)
+void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt = {},
+                    bool after_pre = false);
 // after positions of trajectory t are replaced from the host (mixed path)
 void launch_refresh_posf(const DevPtrs &d, int t, int natom, hipStream_t s);
 // per_atom: also the reference-order per-atom outputs (o_cl, o_cs, o_cek, o_cgs,

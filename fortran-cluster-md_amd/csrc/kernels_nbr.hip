@@ -755,14 +755,17 @@ constexpr int REBUILD_LOOP_GRID = 2048;
 // measured best: rebuild -2.5 us a step against 1024).
 constexpr int CELLSORT_GRID = 8192;
 
-void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt)
+void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s, const KTimer &kt,
+                    bool after_pre)
 {
     const KTimer first{kt.t0, nullptr}, last{nullptr, kt.t1};
     const dim3 blk(BLK);
-    const int wg = g.wl_grid;  // workgroups per rebuild kernel (a tunable)
+    const int wg = g.wl_grid;  // workgroups of the cell-count scans (a tunable)
     const dim3 gc(xcd_grid(std::min(g.ncblk, wg)));
     const dim3 gj(xcd_grid(std::min(g.ntraj, wg)));
-    launch_timed(k_wl_fill, dim3(g.ntraj), blk, s, first, d, g.ntraj);
+    // a single trajectory's k_traj_pre has written the work lists already
+    const bool lists = after_pre && g.ntraj == 1;
+    if (!lists) launch_timed(k_wl_fill, dim3(g.ntraj), blk, s, first, d, g.ntraj);
     // The list search: one workgroup per possible atom block (it needs the
     // registers a loop over blocks would keep).  The binning, the scatter and
     // the permutation: REBUILD_LOOP_GRID workgroups looping over the rebuilt
@@ -771,7 +774,7 @@ void launch_rebuild(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream
     // of the arena that has nothing to do.
     const dim3 gall(xcd_grid(g.nblk));
     const dim3 gl(xcd_grid(std::min(g.nblk, REBUILD_LOOP_GRID)));
-    hipLaunchKernelGGL(k_bin, gl, blk, 0, s, d);
+    launch_timed(k_bin, gl, blk, s, lists ? first : KTimer{}, d);
     hipLaunchKernelGGL(k_scan_chunk, gc, blk, 0, s, d);
     hipLaunchKernelGGL(k_scan_totals, gj, blk, 0, s, d);
     hipLaunchKernelGGL(k_scatter, gl, blk, 0, s, d);

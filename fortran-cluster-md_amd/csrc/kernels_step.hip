@@ -255,16 +255,34 @@ __global__ __launch_bounds__(BLK) void k_drift(DevPtrs d)
 }
 
 // one block per trajectory, after the drift
-__global__ __launch_bounds__(BLK) void k_traj_pre(DevPtrs d)
+// lists (a single trajectory): the block also writes the rebuild's work
+// lists, which k_wl_fill would otherwise make in a launch of its own (every
+// block of the trajectory, every cell chunk, the trajectory itself; empty
+// when it keeps its list)
+__global__ __launch_bounds__(BLK) void k_traj_pre(DevPtrs d, int lists)
 {
     __shared__ double lds[2][4];
     const int t = blockIdx.x;
     const TrajParams &tp = d.tp[t];
     TrajScalars &ts = d.ts[t];
-    if (ts.err) return;
-    double r[2];
-    reduce_partials<2>(tp, d.part, {P_SUMV2, P_DMAX}, {false, true}, lds, r);
-    if (threadIdx.x == 0) traj_pre(tp, ts, r[1], r[0]);
+    const bool err = ts.err;
+    if (!err) {
+        double r[2];
+        reduce_partials<2>(tp, d.part, {P_SUMV2, P_DMAX}, {false, true}, lds, r);
+        if (threadIdx.x == 0) traj_pre(tp, ts, r[1], r[0]);
+    }
+    if (!lists) return;
+    __syncthreads();  // thread 0's rebuild decision
+    const bool rb = ts.rebuild && !ts.err;
+    const int nb = rb ? tp.nblk : 0, nc = rb ? tp.ncblk : 0;
+    for (int q = threadIdx.x; q < nb; q += BLK) d.wl_blk[q] = tp.bfirst + q;
+    for (int q = threadIdx.x; q < nc; q += BLK) d.wl_cchk[q] = tp.cfirst + q;
+    if (threadIdx.x == 0) {
+        d.wl_traj[0] = t;
+        d.wl_cnt[WL_BLK] = nb;
+        d.wl_cnt[WL_CCHK] = nc;
+        d.wl_cnt[WL_TRAJ] = rb ? 1 : 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -982,7 +1000,7 @@ void launch_drift_atoms(const DevPtrs &d, const LaunchDims &g, bool mixed, hipSt
 
 void launch_traj_pre(const DevPtrs &d, const LaunchDims &g, hipStream_t s, const KTimer &kt)
 {
-    launch_timed(k_traj_pre, dim3(g.ntraj), dim3(BLK), s, kt, d);
+    launch_timed(k_traj_pre, dim3(g.ntraj), dim3(BLK), s, kt, d, (int)(g.ntraj == 1));
 }
 
 void launch_drift(const DevPtrs &d, const LaunchDims &g, bool mixed, hipStream_t s)
