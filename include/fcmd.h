@@ -232,7 +232,8 @@ int fcmd_time_kernels(fcmd_system *s, int nsteps, double *avg_ms);
 
 /* Launch tunables: tuning[3] = {neighbours per load batch in the force
  * kernel (2, 4, 8), candidates per load batch in the list search (2, 4),
- * workgroups per rebuild kernel (1024, 2048, 4096)}.  No tunable changes any
+ * workgroups of the list rebuild's cell-count scans (1024, 2048, 4096)}.  No
+ * tunable changes any
  * result bit.  Defaults {4, 2, 2048}.  No counterpart in the reference (its
  * loops have no launch shape). */
 int fcmd_set_tuning(fcmd_system *s, const int *tuning);
