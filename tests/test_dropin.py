@@ -88,6 +88,28 @@ def test_dropin_mav_nve(tmp_path, resident):
     assert _rebuild_lines(ref["stdout"]) == _rebuild_lines(hip["stdout"])
 
 
+@pytest.mark.skipif(not (_have("cluster_mav") and _have("cluster_mav_hip")), reason="programs not built")
+@pytest.mark.parametrize("precision", ["mixed", "kahan"])
+def test_dropin_mav_mixed_precision(tmp_path, precision):
+    """The drop-in run in a mixed form (FCMD_PRECISION): the first cluster
+    histogram (the start state, before the forms part) equals the
+    reference's, and over 50 steps the energies stay within 1e-5 of the
+    reference program's."""
+    case = decks.deck_units(cases.make_case(n=1000, rho=0.02, seed=21, dense_frac=0.3, ncenters=4))
+    ref, hip = _run_pair(tmp_path, "mav", case, ntim=50, ftcsi=50, ftave=10,
+                         env={"FCMD_PRECISION": precision})
+    st0 = min(ref["csi"])
+    assert sorted(ref["csi"][st0]) == sorted(hip["csi"][st0])
+    for size, (count, ke) in ref["csi"][st0].items():
+        assert hip["csi"][st0][size][0] == count
+        assert hip["csi"][st0][size][1] == pytest.approx(ke, rel=1e-9, abs=1e-14)
+    assert len(ref["ave"]) == len(hip["ave"]) > 0
+    for (s1, k1, p1), (s2, k2, p2) in zip(ref["ave"], hip["ave"]):
+        assert s1 == s2
+        assert k2 == pytest.approx(k1, rel=1e-5)
+        assert p2 == pytest.approx(p1, rel=1e-5)
+
+
 @pytest.mark.skipif(not (_have("cluster_nose") and _have("cluster_nose_hip")), reason="programs not built")
 def test_dropin_nose_nh(tmp_path):
     case = decks.deck_units(cases.make_case(n=800, rho=0.03, seed=22, dense_frac=0.3, ethst_nh=1))
